@@ -1,0 +1,248 @@
+"""Pins the CPU oracle (oracle/) -- the checker the HIP path is compared to.
+
+Pins, in order of strength:
+  * golden fixtures written by the reference itself (tests/golden/):
+    CSR arrays from gencrs, probabilities printed with %.20g (exact doubles),
+    densities/covariances printed with %lf, stop step and final time;
+  * the reference's own functions compiled from source (oracle/_ref), where
+    present;
+  * mathematics: closed forms (nnz, row length), column sums of a generator,
+    exact stationary states, the F1 identity k1 = k2 = k3 = k4.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import _golden as G
+
+
+# ----------------------------------------------------------------- CSR (a1)
+@pytest.mark.parametrize("fx", G.csr_fixtures(), ids=lambda f: f["name"])
+def test_gencrs_bit_exact_vs_reference_fixture(fx):
+    rowPtr, colInd, val = oracle.gencrs(fx["n"], fx["rates"])
+    assert np.array_equal(rowPtr, fx["rowPtr"])
+    assert np.array_equal(colInd, fx["colInd"])
+    assert np.array_equal(val.view(np.uint64), fx["val"].view(np.uint64))
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 9, 12])
+def test_nnz_and_rowlen_closed_forms(n):
+    r = np.linspace(0.3, 1.7, n + 1)
+    rowPtr, colInd, val = oracle.gencrs(n, r)
+    dim = 1 << n
+    assert int(rowPtr[-1]) == oracle.nnz(n) == 2 * dim + (n - 1) * dim // 4
+    s = np.arange(dim, dtype=np.uint64)
+    m = (s & ~(s >> np.uint64(1))) & np.uint64((1 << (n - 1)) - 1)
+    pc = np.array([bin(int(v)).count("1") for v in m])
+    rl = 1 + ((s >> np.uint64(n - 1)) & np.uint64(1)).astype(int) + (1 - (s & np.uint64(1)).astype(int)) + pc
+    assert np.array_equal(np.diff(rowPtr).astype(int), rl)
+    for st in (0, 1, dim // 2, dim - 1, dim // 3):
+        assert oracle.rowlen(n, st) == rl[st]
+    # columns ascending within each row (entry < diag < exit < bulk)
+    for i in range(0, dim, max(1, dim // 64)):
+        cols = colInd[rowPtr[i]:rowPtr[i + 1]]
+        assert np.all(np.diff(cols.astype(np.int64)) > 0)
+
+
+@pytest.mark.parametrize("n", [3, 6, 8])
+def test_generator_columns_sum_to_zero(n):
+    r = np.random.default_rng(n).uniform(0.1, 2.0, n + 1)
+    L = oracle.dense(n, r)
+    assert np.max(np.abs(L.sum(axis=0))) < 1e-14
+    assert np.all(np.diag(L) <= 0)
+    off = L - np.diag(np.diag(L))
+    assert np.all(off >= 0)
+
+
+def test_mf_row_equals_csr_row():
+    n = 7
+    r = np.random.default_rng(1).uniform(0.1, 2.0, n + 1)
+    rowPtr, colInd, val = oracle.gencrs(n, r)
+    for s in range(1 << n):
+        c, v = oracle.row(n, s, r)
+        assert np.array_equal(c, colInd[rowPtr[s]:rowPtr[s + 1]])
+        assert np.array_equal(v, val[rowPtr[s]:rowPtr[s + 1]])
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="oracle/_ref not built here")
+def test_gencrs_and_rk4_step_vs_reference_library():
+    import ctypes as C
+    R = oracle.ref_lib()
+    n = 9
+    dim = 1 << n
+    r = np.random.default_rng(3).uniform(0.1, 2.0, n + 1)
+    rp = np.zeros(dim + 1, np.uint64)
+    ci = np.zeros(oracle.nnz(n), np.uint64)
+    va = np.zeros(oracle.nnz(n), np.float64)
+    import os
+    import sys
+    sys.stdout.flush()
+    saved = os.dup(1)
+    dn = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(dn, 1)
+    try:
+        R.gencrs(rp, va, ci, r, np.zeros(n, np.int32), dim, n)
+        C.CDLL(None).fflush(None)
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(dn)
+    rowPtr, colInd, val = oracle.gencrs(n, r)
+    assert np.array_equal(rp, rowPtr) and np.array_equal(ci, colInd)
+    assert np.array_equal(va.view(np.uint64), val.view(np.uint64))
+    x = np.random.default_rng(4).random(dim)
+    x /= x.sum()
+    k_ref = np.zeros(dim)
+    x1 = np.zeros(dim)
+    R.rk4_step(1, dim, k_ref, x, x1, rp, ci, va, 0.01)
+    k = oracle.apply_csr(rowPtr, colInd, val, x, 0.01)
+    assert np.array_equal(k.view(np.uint64), k_ref.view(np.uint64))
+    assert np.array_equal(oracle.apply_mf(n, r, x, 0.01).view(np.uint64),
+                          k.view(np.uint64))
+
+
+# ------------------------------------------------------- step loop (a3-a6)
+CLI = G.cli_case_names()
+
+
+@pytest.mark.parametrize("name", CLI)
+def test_run_ref_bit_exact_vs_reference_cli(name):
+    c = G.cli_case(name)
+    n = c["n"]
+    h = oracle.step_h(c["T"], c["t0"], c["steps"])
+    x0 = oracle.uniform_x0(n)
+    if c["tv"] is not None:
+        x, info = oracle.run_ref_tv(n, x0, c["t0"], h, c["steps"], c["tv"])
+    else:
+        x, info = oracle.run_ref(n, c["rates"], x0, c["t0"], h, c["steps"],
+                                 use_csr=c["csr"])
+    assert np.array_equal(x.view(np.uint64), c["x"].view(np.uint64)), \
+        f"max|dx|={np.max(np.abs(x - c['x']))}"
+    conv = G.stdout_value(c, "Convergence after ")
+    if conv is not None:
+        assert info.stopped or info.early_stop > 0
+        assert int(conv.split()[0]) == info.steps_at_stop
+    else:
+        assert info.early_stop == 0
+    assert G.stdout_value(c, "Simulation time elapsed:") == "%f" % info.t
+    assert G.stdout_value(c, "Checksum over result vector:") == "%f" % oracle.mass(x)
+
+
+@pytest.mark.parametrize("name", CLI)
+def test_moments_format_vs_reference_cli(name):
+    c = G.cli_case(name)
+    n = c["n"]
+    y = oracle.site_moments(n, c["x"])
+    vec = ", ".join(["%f" % y[n - 1 - i] for i in range(n)])
+    assert G.read_text(c, "out_exp_values_fin.vec") == vec
+    dat = "".join("%d Pos%d %f\n" % (i, i, y[n - 1 - i]) for i in range(n))
+    assert G.read_text(c, "out_exp_values_fin.dat") == dat
+    cov, mul = oracle.cov(n, c["x"], y)
+    mat = "".join(", ".join("%f" % v for v in row) + "\n" for row in cov)
+    assert G.read_text(c, "out_cov.mat") == mat
+
+
+def test_ref_stages_are_equal_F1():
+    """All four reference stages read the unmodified x: with the reference's
+    own rk4_step, k1..k4 are bitwise equal (survey finding F1)."""
+    if not oracle.have_ref():
+        pytest.skip("oracle/_ref not built here")
+    R = oracle.ref_lib()
+    n = 6
+    dim = 1 << n
+    r = np.random.default_rng(9).uniform(0.1, 2.0, n + 1)
+    rp, ci, va = oracle.gencrs(n, r)
+    x = np.random.default_rng(2).random(dim)
+    x1 = np.zeros(dim)
+    ks = []
+    for st in (1, 2, 3, 4):
+        k = np.zeros(dim)
+        R.rk4_step(st, dim, k, x, x1, rp, ci, va, 0.05)
+        ks.append(k)
+    for k in ks[1:]:
+        assert np.array_equal(k, ks[0])
+
+
+# ------------------------------------------------------------ moments (a7-a9)
+def test_site_moments_and_cov_identities():
+    n = 7
+    x = np.random.default_rng(11).random(1 << n)
+    x /= x.sum()
+    y = oracle.site_moments(n, x)
+    s = np.arange(1 << n)
+    bits = ((s[:, None] >> np.arange(n)[None, :]) & 1).astype(float)
+    assert np.allclose(y, bits.T @ x, rtol=0, atol=1e-15)
+    cov, mul = oracle.cov(n, x, y)
+    E = (bits * x[:, None]).T @ bits
+    M = x.sum()
+    ident = E - np.outer(y, y) * (2 - M)      # exact identity for the centred sum
+    # written orientation: cov[n-1-i][n-1-k]
+    assert np.allclose(cov[::-1, ::-1], ident, rtol=0, atol=1e-14)
+    assert np.allclose(mul, cov + np.outer(y, y)[::-1, ::-1], rtol=0, atol=1e-15)
+
+
+# ------------------------------------------------------- rk4 + steady state
+def test_rk4_oracle_matches_expm_at_fourth_order():
+    from scipy.linalg import expm
+    n = 5
+    r = np.random.default_rng(5).uniform(0.2, 1.5, n + 1)
+    L = oracle.dense(n, r)
+    x0 = np.zeros(1 << n)
+    x0[0] = 1.0
+    exact = expm(1.0 * L) @ x0
+    errs = []
+    for steps in (10, 20, 40):
+        x, _ = oracle.run_rk4(n, r, x0, 0.0, 1.0 / steps, steps)
+        errs.append(np.max(np.abs(x - exact)))
+    # fourth order: halving h divides the error by ~16
+    assert errs[0] / errs[1] > 12 and errs[1] / errs[2] > 12
+    assert errs[2] < 1e-6
+
+
+def test_ref_loop_is_explicit_euler():
+    """F1: the reference update equals x + h L x (up to its 1-2-2-1 rounding)."""
+    n = 5
+    r = np.random.default_rng(6).uniform(0.2, 1.5, n + 1)
+    L = oracle.dense(n, r)
+    x0 = oracle.uniform_x0(n)
+    x, _ = oracle.run_ref(n, r, x0, 0.0, 0.01, 1, patience=-1)
+    assert np.allclose(x, x0 + 0.01 * (L @ x0), rtol=0, atol=1e-17)
+
+
+def test_uniform_is_stationary_for_half_half_unit():
+    """alpha = beta = 1/2, gamma = 1: the uniform state is exactly stationary
+    (the reference stops after 101 sub-threshold steps)."""
+    n = 6
+    r = np.array([0.5] + [1.0] * (n - 1) + [0.5])
+    x, info = oracle.run_ref(n, r, oracle.uniform_x0(n), 0.0, 0.01, 10000)
+    assert info.stopped and info.steps_at_stop == 100 and info.steps_done == 101
+    assert np.all(x == 1.0 / 64)
+
+
+def test_dehp_matches_exact_bruteforce():
+    from fractions import Fraction as F
+    from oracle import dehp
+    for n in (1, 2, 3, 4):
+        for a, b in ((0.3, 0.7), (F(1, 3), F(3, 4)), (1.5, 0.25)):
+            p = dehp.steady_state_bruteforce(n, [a] + [1] * (n - 1) + [b])
+            assert dehp.densities(n, a, b) == dehp.densities_from_p(n, p)
+
+
+def test_converged_ref_run_matches_dehp():
+    from oracle import dehp
+    n = 8
+    a, b = 0.3, 0.6
+    r = np.array([a] + [1.0] * (n - 1) + [b])
+    x, info = oracle.run_ref(n, r, oracle.uniform_x0(n), 0, 0.02, 40000, thr=1e-10)
+    assert info.stopped
+    y = oracle.site_moments(n, x)[::-1]          # entry .. exit
+    rho = np.array([float(v) for v in dehp.densities(n, a, b)])
+    assert np.max(np.abs(y - rho)) < 1e-6
+
+
+def test_gamma_rescaled_twin():
+    from oracle import dehp
+    n = 4
+    r2 = [0.6] + [2.0] * (n - 1) + [0.8]
+    p = dehp.steady_state_bruteforce(n, r2)
+    assert dehp.densities_from_p(n, p) == dehp.densities(n, 0.6, 0.8, 2.0)
