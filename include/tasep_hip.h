@@ -1,0 +1,153 @@
+/*
+ * tasep_hip.h -- C ABI of the MI355X (gfx950) master-equation engine.
+ *
+ * Drop-in for the compute section of KilianPiochUBT/TASEP's masterEquation:
+ * the generator build (gencrs), the stage kernels (rk4_step,
+ * gen_meq_row + rk4_step_reduced), the update/checksum/early-stop loop and
+ * the moment reductions (condense, exp_val_eval, the covariance loops).
+ * The reference calls those as plain C functions on host buffers
+ * (src/lib/matgen.h:12-19, src/lib/solv.h:34-62, src/lib/matvec.h:9-34) from
+ * src/masterEquation.c:586-992 and :1009-1139,1291-1303.  Here the state
+ * vector lives in HBM for the whole run; the host hands over rates and x0
+ * once and reads back n + 2n^2 + 1 doubles (plus x only when asked).
+ *
+ * Conventions (as the reference): state s in [0, 2^n), bit k of s is site
+ * n-k counted from the entry (bit n-1 entry site, bit 0 exit site);
+ * r[0] = alpha, r[n] = beta, r[j] = hop rate from site j to j+1.
+ *
+ * Every call returns 0 on success or a negative TASEP_ERR_* code
+ * (tasep_strerror).  A context is single-threaded: calls on one context must
+ * not overlap.  Each context owns its device memory and one HIP stream of
+ * default priority.
+ */
+#ifndef TASEP_HIP_H
+#define TASEP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TASEP_OK               0
+#define TASEP_ERR_ARG         -1  /* invalid argument / combination          */
+#define TASEP_ERR_NOMEM       -2  /* device or host allocation failed        */
+#define TASEP_ERR_HIP         -3  /* HIP runtime error                       */
+#define TASEP_ERR_STATE       -4  /* call out of order (e.g. no rates set)   */
+#define TASEP_ERR_NODEV       -5  /* no usable gfx950 device                 */
+
+/* precision of the state vector */
+#define TASEP_F64   0  /* FP64 vector, FP64 arithmetic                        */
+#define TASEP_F32   1  /* FP32 vector; generator sums, stage combination,
+                          checksum, mass and moments in FP64                  */
+/* integrator */
+#define TASEP_REF   0  /* the reference's loop: all four stages read the
+                          unmodified x (src/lib/solv.c:12-13), update
+                          x += (((k + 2k) + 2k) + k)/6, bit-faithful in FP64  */
+#define TASEP_RK4   1  /* classical fourth-order Runge-Kutta                  */
+/* generator */
+#define TASEP_MF    0  /* matrix-free: rows derived from the state bitmask    */
+#define TASEP_CSR   1  /* stored CSR built on the device (the -f path)        */
+
+typedef struct tasep_ctx tasep_ctx;
+
+typedef struct {
+    uint64_t steps_done;     /* steps executed by this call                   */
+    uint64_t steps_at_stop;  /* index of the last sub-threshold step          */
+    int32_t  early_stop;     /* consecutive sub-threshold steps at the end    */
+    int32_t  stopped;        /* 1: left the loop on early_stop > patience     */
+    double   t;              /* time after the last step (t0 + h + h + ...)   */
+    double   last_checksum;  /* sum_s |dx_s| of the last step                 */
+    double   mass;           /* sum_s x_s after the last step (FP64 sum)      */
+} tasep_run_info;
+
+/* Create a context for a lattice of n sites (2 <= n <= 34).
+ * Valid combinations: REF requires F64; CSR requires F64 and n <= 30.
+ * Replaces the allocations of src/masterEquation.c:349,472-505,593-614. */
+int tasep_ctx_create(tasep_ctx **ctx, int n, int precision, int integrator,
+                     int generator);
+int tasep_ctx_destroy(tasep_ctx *ctx);
+
+/* n+1 constant jump rates (src/masterEquation.c:432-461). */
+int tasep_set_rates(tasep_ctx *ctx, const double *r);
+
+/* Time-variant rates r_i(t) = max(0, A_i*f(w_i*t + phi_i) + c_i), f = sin
+ * (type 0), cos (type 1) or A_i*t + c_i (type 2); each array has n+1
+ * entries.  Replaces timeVariantProbs (src/lib/matvec.c:44-68), evaluated on
+ * the device in FP64 once per step at the step's start time for REF
+ * (src/masterEquation.c:886) and at t, t+h/2, t+h for RK4.  MF only. */
+int tasep_set_time_variant(tasep_ctx *ctx, const int32_t *type,
+                           const double *amp, const double *period,
+                           const double *offset, const double *c);
+
+/* Initial states: uniform 2^-n (src/masterEquation.c:402-409), a single
+ * occupied basis state e_s, or an arbitrary host FP64 vector of 2^n. */
+int tasep_set_x0_uniform(tasep_ctx *ctx);
+int tasep_set_x0_state(tasep_ctx *ctx, uint64_t s);
+int tasep_set_x(tasep_ctx *ctx, const double *x);
+/* Copy the current state to a host FP64 vector of 2^n entries. */
+int tasep_get_x(tasep_ctx *ctx, double *x);
+
+/* Build the CSR generator on the device (replaces gencrs,
+ * src/lib/matgen.c:7-74; arrays bit-identical to it).  Requires rates. */
+int tasep_build_csr(tasep_ctx *ctx);
+/* Copy the device CSR to host arrays sized 2^n+1 / tasep_csr_nnz(n). */
+int tasep_get_csr(tasep_ctx *ctx, uint64_t *rowPtr, uint64_t *colInd,
+                  double *val);
+/* nnz = 2*2^n + (n-1)*2^(n-2)  (src/masterEquation.c:593). */
+uint64_t tasep_csr_nnz(int n);
+
+/* Same signature meaning as the reference's gencrs (src/lib/matgen.h:12-19):
+ * fills host rowPtr[dim+1], valArr[nnz], colInd[nnz] from jump_rate_Array,
+ * building on the device.  No scratch `bit` array is needed. */
+int tasep_gencrs(uint64_t *rowPtr, double *valArr, uint64_t *colInd,
+                 const double *jump_rate_Array, uint64_t dim, int n);
+
+/* Run `steps` steps of size h from time t0 (replaces the loops at
+ * src/masterEquation.c:628-682, :883-921, :929-982).  With constant rates the
+ * loop leaves after more than `patience` consecutive steps whose checksum
+ * sum_s|dx_s| is below thr*h (reference: thr = 1e-7, patience = 100,
+ * :186,670-679); patience < 0 disables the exit.  Time-variant runs never
+ * exit early (as the reference).  The state stays on the device. */
+int tasep_run(tasep_ctx *ctx, double t0, double h, uint64_t steps, double thr,
+              int patience, tasep_run_info *info);
+
+/* Moments of the current state in one device pass with FP64 sums
+ * (replaces condense/exp_val_eval, src/lib/matvec.c:10-91, and the two
+ * O(n^2 2^n) loops at src/masterEquation.c:1127-1139,1291-1303):
+ *   y[i]       = sum_s b_i(s) x_s                  (i = bit, 0 = exit site)
+ *   cov[i*n+k] = sum_s (b_k - y_k)(b_i - y_i) x_s
+ *   mul[i*n+k] = y_i y_k + cov[i*n+k]
+ *   *mass      = sum_s x_s
+ * For TASEP_F32 contexts x is first divided by its FP64 mass.  Any output
+ * pointer may be NULL.  Index by bit; the reference writes the matrices
+ * with both axes reversed (entry site first). */
+int tasep_moments(tasep_ctx *ctx, double *y, double *cov, double *mul,
+                  double *mass);
+
+/* Per-launch timing of one step, for measurement: runs `reps` steps from t0
+ * (advancing the state like tasep_run with patience < 0), timing every
+ * kernel launch of a step with HIP events on the context's stream, and
+ * returns for each launch position its name, mean duration and algorithmic
+ * HBM bytes per state.  No reference counterpart (the reference times whole
+ * phases with gettimeofday, src/masterEquation.c:589,618-626,993-1001). */
+typedef struct {
+    char   name[32];
+    double ms;
+    double bytes_per_state;
+} tasep_kernel_time;
+int tasep_step_profile(tasep_ctx *ctx, double t0, double h, int reps,
+                       tasep_kernel_time *out, int max, int *count);
+
+/* The context's HIP stream (hipStream_t) and device pointer of the current
+ * state vector (double* or float* by precision), for callers that time or
+ * consume the state on the device. */
+int tasep_get_stream(tasep_ctx *ctx, void **stream);
+int tasep_get_x_device(tasep_ctx *ctx, void **ptr);
+
+const char *tasep_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TASEP_HIP_H */

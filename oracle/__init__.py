@@ -86,6 +86,8 @@ def lib():
         L.tor_mass.restype = C.c_double
         L.tor_mass.argtypes = [C.c_uint64, _f64p]
         L.tor_dense.argtypes = [C.c_int, _f64p, _f64p]
+        L.tor_ref_update4.restype = C.c_double
+        L.tor_ref_update4.argtypes = [C.c_uint64, _f64p, _f64p, _f64p, _f64p, _f64p]
         _lib = L
     return _lib
 

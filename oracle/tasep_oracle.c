@@ -332,6 +332,17 @@ double tor_mass(uint64_t dim, const double *x)
     return m;
 }
 
+double tor_ref_update4(uint64_t dim, double *x, const double *k1,
+                       const double *k2, const double *k3, const double *k4)
+{
+    double chk = 0;
+    for (uint64_t s = 0; s < dim; s++) {   /* masterEquation.c:639-642 */
+        x[s] += (k1[s] + (2. * k2[s]) + (2. * k3[s]) + k4[s]) / 6.;
+        chk += fabs((k1[s] + (2. * k2[s]) + (2. * k3[s]) + k4[s]) / 6.);
+    }
+    return chk;
+}
+
 void tor_dense(int n, const double *r, double *L)
 {
     uint64_t dim = (uint64_t)1 << n;

@@ -106,6 +106,12 @@ void tor_cov(int n, const double *x, const double *y, double *cov, double *mul);
 /* Sequential sum of x (src/masterEquation.c:1009-1011). */
 double tor_mass(uint64_t dim, const double *x);
 
+/* The reference's update with four stage arrays, as written at
+ * src/masterEquation.c:639-642; returns the checksum sum |d|.  (CPU-baseline
+ * timing pairs it with the reference's own rk4_step.) */
+double tor_ref_update4(uint64_t dim, double *x, const double *k1,
+                       const double *k2, const double *k3, const double *k4);
+
 /* Dense generator L[s*dim + c] for small n (tests: column sums, brute force).*/
 void tor_dense(int n, const double *r, double *L);
 

@@ -1,0 +1,128 @@
+// device_common.hpp -- wave/block reductions, run state, rate staging.
+//
+// Wavefront size is 64 on gfx950; every kernel here is launched with
+// blockDim.x == kBlock (4 waves).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tasep {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+constexpr int kWaves = kBlock / kWave;
+constexpr int kMaxN = 34;
+constexpr int kRateStride = 40;  // doubles per rate set (n+1 <= 35)
+
+// Device-resident loop state.  Written only by the last block of a step's
+// final kernel (and by the host between runs); read by every kernel at entry.
+struct RunState {
+    double t;            // time after the last finished step
+    double h;
+    double thr_h;        // early-exit threshold thr*h
+    double checksum;     // sum |dx| of the last step
+    double mass;         // sum x after the last step
+    double scale;        // FP32 path: 1/mass of the stored vector (else 1)
+    uint64_t steps_done;
+    uint64_t steps_at_stop;
+    int32_t early_stop;
+    int32_t patience;    // < 0: never leave early
+    int32_t stopped;
+    int32_t renorm;      // 1: scale <- 1/mass after each step
+    uint32_t counter;    // blocks of the current kernel that have finished
+    uint32_t pad;
+};
+
+// Neumaier-compensated FP64 accumulator.
+struct Kahan {
+    double s = 0.0, c = 0.0;
+    __device__ __forceinline__ void add(double v) {
+        double t = s + v;
+        if (fabs(s) >= fabs(v)) c += (s - t) + v;
+        else c += (v - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return s + c; }
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Fixed-shape block sum (deterministic for a fixed launch shape).  Result is
+// valid in every thread.  `red` must hold kWaves doubles.
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < kWaves; i++) r += red[i];
+    return r;
+}
+
+__device__ __forceinline__ double load_relaxed(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// End-of-step bookkeeping, called by every block of a step's final kernel
+// with its block partials of sum|dx| and sum x.  The last block to arrive
+// reduces all partials in a fixed order (so the result does not depend on
+// which block is last) and advances the run state exactly as the reference
+// loop does (src/masterEquation.c:644,670-679): t += h, then the
+// sub-threshold counter and the exit test.
+__device__ void finish_step(RunState *st, double *part_chk, double *part_mass,
+                            double blk_chk, double blk_mass) {
+    __shared__ int s_last;
+    __shared__ double s_red[kWaves];
+    if (threadIdx.x == 0) {
+        part_chk[blockIdx.x] = blk_chk;
+        part_mass[blockIdx.x] = blk_mass;
+        __threadfence();
+        uint32_t prev = atomicAdd(&st->counter, 1u);
+        s_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    Kahan kc, km;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+        kc.add(load_relaxed(part_chk + b));
+        km.add(load_relaxed(part_mass + b));
+    }
+    double chk = block_sum(kc.value(), s_red);
+    double mass = block_sum(km.value(), s_red);
+    if (threadIdx.x == 0) {
+        st->checksum = chk;
+        st->mass = mass;
+        st->t = st->t + st->h;
+        uint64_t i = st->steps_done;       // index of this step in the run
+        st->steps_done = i + 1;
+        if (st->patience >= 0) {
+            if (chk < st->thr_h) {
+                st->early_stop += 1;
+                st->steps_at_stop = i;
+            } else {
+                st->early_stop = 0;
+            }
+            if (st->early_stop > st->patience) st->stopped = 1;
+        }
+        if (st->renorm && mass > 0.0) st->scale = 1.0 / mass;
+        st->counter = 0u;
+        __threadfence();
+    }
+}
+
+// Stage the rate set `set` (n+1 doubles) into LDS.
+__device__ __forceinline__ void stage_rates(double *lds, const double *rates,
+                                            int set, int n) {
+    if (threadIdx.x <= n) lds[threadIdx.x] = rates[set * kRateStride + threadIdx.x];
+    __syncthreads();
+}
+
+}  // namespace tasep

@@ -1,0 +1,374 @@
+// kernels.hpp -- the step, stage, CSR-build, rate and moment kernels.
+#pragma once
+
+#include "device_common.hpp"
+#include "generator.hpp"
+
+namespace tasep {
+
+// ===========================================================================
+// Reference loop (TASEP_REF): one kernel per step.  Because the reference's
+// four stages all read the unmodified x, one generator pass gives k and the
+// update x_new = x + (((k + 2k) + 2k) + k)/6 is fused into its epilogue
+// (16 B of HBM per state: read x, write x_new; neighbours hit in cache).
+// ===========================================================================
+struct RefArgs {
+    const double *x;
+    double *xn;
+    const double *rates;
+    const uint64_t *rowPtr;
+    const uint64_t *colInd;
+    const double *val;
+    RunState *st;
+    double *pchk;
+    double *pmass;
+    uint64_t dim;
+    double h;
+    Lattice g;
+};
+
+template <bool CSR>
+__global__ __launch_bounds__(kBlock) void k_ref_step(const RefArgs a) {
+    if (a.st->stopped) return;
+    __shared__ double r[kRateStride];
+    __shared__ double red[kWaves];
+    stage_rates(r, a.rates, 0, a.g.n);
+    Kahan chk, mass;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t s = (uint64_t)blockIdx.x * kBlock + threadIdx.x; s < a.dim; s += stride) {
+        double T = CSR ? ref_row_csr(s, a.rowPtr, a.colInd, a.val, a.x)
+                       : ref_row_mf(a.g, s, a.x, r);
+        double d = ref_delta(a.h, T);
+        double xn = a.x[s] + d;
+        a.xn[s] = xn;
+        chk.add(fabs(d));
+        mass.add(xn);
+    }
+    double bc = block_sum(chk.value(), red);
+    double bm = block_sum(mass.value(), red);
+    finish_step(a.st, a.pchk, a.pmass, bc, bm);
+}
+
+// ===========================================================================
+// Classical RK4 passes (TASEP_RK4).  One kernel template covers both forms:
+//
+//  staged (any rates, incl. time-variant):
+//    S1: K1 = h L(t) X            S2: K2 = h L(t+h/2) (X + K1/2)
+//    S3: K3 = h L(t+h/2)(X + K2/2) S4: XN = X + (K1 + 2K2 + 2K3 + h L(t+h)(X+K3))/6
+//  Horner (constant rates; the same update, since for x' = Lx one RK4 step
+//  is x <- (1 + hL + (hL)^2/2 + (hL)^3/6 + (hL)^4/24) x):
+//    P1: U = X + h/4 L X   P2: V = X + h/3 L U   P3: U = X + h/2 L V
+//    P4: X = X + h L U  (in place: X is read only at s)
+//
+// The gathered stage input is G(j) = gs*GX[j] + ck*GK[j]; the self term is
+// sc*X[s], where sc = RunState::scale renormalises FP32 vectors by the FP64
+// mass of the previous step.  All arithmetic is FP64; V is the storage type.
+// ===========================================================================
+enum { EPI_STORE = 0, EPI_HORNER = 1, EPI_RK4 = 2 };
+
+template <class V>
+struct PassArgs {
+    const V *gx;      // gathered vector
+    const V *gk;      // optional second gathered vector
+    double ck;        // weight of gk
+    int gx_is_x;      // 1: gx is the state (scaled by sc)
+    int use_self;     // STORE: include a_self*sc*X[s]
+    const V *x;       // state (self term)
+    V *out;
+    const V *k1, *k2, *k3;
+    double c;         // multiplier of (L G)(s)
+    const double *rates;
+    int rset;
+    const uint64_t *rowPtr;
+    const uint64_t *colInd;
+    const double *val;
+    RunState *st;
+    double *pchk;
+    double *pmass;
+    uint64_t dim;
+    Lattice g;
+};
+
+template <class V>
+struct Gather1 {
+    const V *__restrict__ p;
+    double w;
+    __device__ __forceinline__ double operator()(uint64_t j) const { return w * (double)p[j]; }
+};
+
+template <class V>
+struct Gather2 {
+    const V *__restrict__ p;
+    const V *__restrict__ q;
+    double w, ck;
+    __device__ __forceinline__ double operator()(uint64_t j) const {
+        return __builtin_fma(ck, (double)q[j], w * (double)p[j]);
+    }
+};
+
+// SELF: the self term is read from the gathered vector (Horner P1, where the
+// gathered vector is the state itself), so the pass streams 8 B/state in FP32.
+template <class V, bool CSR, bool TWO, int EPI, bool SELF = false>
+__global__ __launch_bounds__(kBlock) void k_rk_pass(const PassArgs<V> a) {
+    if (a.st->stopped) return;
+    __shared__ double r[kRateStride];
+    __shared__ double red[kWaves];
+    stage_rates(r, a.rates, a.rset, a.g.n);
+    const double sc = a.st->scale;
+    const double w = a.gx_is_x ? sc : 1.0;
+    Kahan chk, mass;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t s = (uint64_t)blockIdx.x * kBlock + threadIdx.x; s < a.dim; s += stride) {
+        double acc;
+        if constexpr (TWO) {
+            Gather2<V> G{a.gx, a.gk, w, a.ck};
+            acc = CSR ? rk_row_csr(s, a.rowPtr, a.colInd, a.val, G) : rk_row_mf(a.g, s, G, r);
+        } else {
+            Gather1<V> G{a.gx, w};
+            acc = CSR ? rk_row_csr(s, a.rowPtr, a.colInd, a.val, G) : rk_row_mf(a.g, s, G, r);
+        }
+        if constexpr (EPI == EPI_STORE) {
+            double o = a.c * acc;
+            if constexpr (SELF) o += sc * (double)a.gx[s];
+            else if (a.use_self) o += sc * (double)a.x[s];
+            a.out[s] = (V)o;
+        } else {
+            double d;
+            if constexpr (EPI == EPI_HORNER) {
+                d = a.c * acc;
+            } else {
+                const double k4 = a.c * acc;
+                d = (((double)a.k1[s] + 2.0 * (double)a.k2[s]) + 2.0 * (double)a.k3[s] + k4) / 6.0;
+            }
+            const V o = (V)(sc * (double)a.x[s] + d);
+            a.out[s] = o;
+            chk.add(fabs(d));
+            mass.add((double)o);
+        }
+    }
+    if constexpr (EPI != EPI_STORE) {
+        double bc = block_sum(chk.value(), red);
+        double bm = block_sum(mass.value(), red);
+        finish_step(a.st, a.pchk, a.pmass, bc, bm);
+    }
+}
+
+// ===========================================================================
+// Time-variant rates, FP64 on the device (replaces timeVariantProbs,
+// src/lib/matvec.c:44-68): r_i(t) = max(0, A_i f(w_i t + phi_i) + c_i).
+// Set 0 at the step's start time, sets 1 and 2 at t + h/2 and t + h.
+// ===========================================================================
+struct TvArgs {
+    const int32_t *type;
+    const double *amp, *period, *offset, *c;
+};
+
+__global__ void k_tv_rates(RunState *st, TvArgs tv, double *rates, int n, int nsets) {
+    if (st->stopped) return;
+    const int i = threadIdx.x;
+    if (i > n) return;
+    const double t0 = st->t, h = st->h;
+    for (int set = 0; set < nsets; set++) {
+        const double t = set == 0 ? t0 : (set == 1 ? t0 + 0.5 * h : t0 + h);
+        double v;
+        switch (tv.type[i]) {
+        case 0: v = tv.amp[i] * sin(t * tv.period[i] + tv.offset[i]) + tv.c[i]; break;
+        case 1: v = tv.amp[i] * cos(t * tv.period[i] + tv.offset[i]) + tv.c[i]; break;
+        default: v = tv.amp[i] * t + tv.c[i]; break;
+        }
+        rates[set * kRateStride + i] = v < 0.0 ? 0.0 : v;
+    }
+}
+
+// ===========================================================================
+// CSR build (replaces gencrs, src/lib/matgen.c:7-74): one lane per row, the
+// row start from its closed form, entries in the reference's order with the
+// diagonal subtracted in the reference's order, so all three arrays are
+// bit-identical.
+// ===========================================================================
+__global__ __launch_bounds__(kBlock) void k_csr_build(int n, const double *__restrict__ rates,
+                                                      uint64_t *__restrict__ rowPtr,
+                                                      uint64_t *__restrict__ colInd,
+                                                      double *__restrict__ val, uint64_t dim) {
+    __shared__ double r[kRateStride];
+    stage_rates(r, rates, 0, n);
+    const Lattice g = make_lattice(n);
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t s = (uint64_t)blockIdx.x * kBlock + threadIdx.x; s < dim; s += stride) {
+        uint64_t p = csr_row_start(n, s);
+        rowPtr[s] = p;
+        if (s == dim - 1) rowPtr[dim] = csr_row_start(n, dim);
+        if (s & g.top) {
+            colInd[p] = s - g.top;
+            val[p] = r[0];
+            p++;
+        }
+        double d = 0.0;
+        if (s & 1) d -= r[n];
+        if (!(s & g.top)) d -= r[0];
+        uint64_t mo = ~s & (s >> 1) & g.bulk;
+        while (mo) {
+            int k = __builtin_ctzll(mo);
+            mo &= mo - 1;
+            d -= r[n - 1 - k];
+        }
+        colInd[p] = s;
+        val[p] = d;
+        p++;
+        if (!(s & 1)) {
+            colInd[p] = s + 1;
+            val[p] = r[n];
+            p++;
+        }
+        uint64_t mi = s & ~(s >> 1) & g.bulk;
+        while (mi) {
+            int k = __builtin_ctzll(mi);
+            mi &= mi - 1;
+            colInd[p] = s + ((uint64_t)1 << k);
+            val[p] = r[n - 1 - k];
+            p++;
+        }
+    }
+}
+
+// ===========================================================================
+// Moments in one pass (replaces condense/exp_val_eval and the two
+// O(n^2 2^n) covariance loops): all E_ik = sum_s b_i b_k x_s (i <= k; E_ii =
+// y_i) and M = sum_s x_s.
+//
+// A block takes rows of 1024 states; lane t holds s = row*1024 + t + 256 j,
+// j = 0..3, so bits 0-7 are the lane, bits 8-9 the register j and bits >= 10
+// the row (uniform).  Per lane A = (S, P8, P9, P89) = (sum_j v_j, v_1+v_3,
+// v_2+v_3, v_3); a superset-sum (zeta) transform over the 8 lane bits -- six
+// rounds of wave shuffles and two through LDS -- leaves in lane m the sums
+// over all lanes u with u & m == m.  Lane m = 0, 2^i, 2^i+2^k then holds the
+// row's mass, site and pair sums for the low bits; pairs that involve bits
+// >= 10 take the row mass (or a low-bit sum) when the row has those bits.
+// Each lane owns <= 3 output quantities, accumulated with Neumaier sums;
+// a second kernel adds the block partials in block order (deterministic).
+// ===========================================================================
+constexpr int kMomRowBits = 10;
+constexpr int kMomMaxQ = kMaxN * (kMaxN + 1) / 2 + 1;  // 596
+constexpr int kMomOwn = (kMomMaxQ + kBlock - 1) / kBlock;
+
+__host__ __device__ inline int tri_index(int i, int k) {  // i <= k
+    return k * (k + 1) / 2 + i;
+}
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_moments_partial(const V *__restrict__ x, int n,
+                                                            uint64_t dim, uint64_t rows,
+                                                            double *__restrict__ part) {
+    __shared__ double zs[4 * kBlock];
+    const int t = threadIdx.x;
+    const int nq = n * (n + 1) / 2 + 1;
+    int qidx[kMomOwn];
+    uint64_t qmask[kMomOwn];
+    Kahan acc[kMomOwn];
+    for (int o = 0; o < kMomOwn; o++) {
+        const int q = t + o * kBlock;
+        qidx[o] = -1;
+        qmask[o] = 0;
+        if (q >= nq) continue;
+        if (q == nq - 1) {  // mass
+            qidx[o] = 0;
+            continue;
+        }
+        int k = 0;
+        while (tri_index(0, k + 1) <= q) k++;
+        const int i = q - tri_index(0, k);
+        int m = 0, p8 = 0, p9 = 0;
+        uint64_t hm = 0;
+        const int bits[2] = {i, k};
+        for (int b = 0; b < (i == k ? 1 : 2); b++) {
+            const int bit = bits[b];
+            if (bit < 8) m |= 1 << bit;
+            else if (bit == 8) p8 = 1;
+            else if (bit == 9) p9 = 1;
+            else hm |= (uint64_t)1 << (bit - kMomRowBits);
+        }
+        qidx[o] = (p8 && p9) ? 3 * kBlock : (p8 ? kBlock + m : (p9 ? 2 * kBlock + m : m));
+        qmask[o] = hm;
+    }
+    for (uint64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const uint64_t base = row << kMomRowBits;
+        double v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t s = base + t + (uint64_t)j * kBlock;
+            v[j] = s < dim ? (double)x[s] : 0.0;
+        }
+        double S = (v[0] + v[1]) + (v[2] + v[3]);
+        double P8 = v[1] + v[3], P9 = v[2] + v[3], P89 = v[3];
+#pragma unroll
+        for (int b = 0; b < 6; b++) {
+            const double oS = __shfl_xor(S, 1 << b, kWave);
+            const double o8 = __shfl_xor(P8, 1 << b, kWave);
+            const double o9 = __shfl_xor(P9, 1 << b, kWave);
+            const double o89 = __shfl_xor(P89, 1 << b, kWave);
+            if (!(t & (1 << b))) {
+                S += oS;
+                P8 += o8;
+                P9 += o9;
+                P89 += o89;
+            }
+        }
+        for (int b = 6; b < 8; b++) {
+            __syncthreads();
+            zs[t] = S;
+            zs[kBlock + t] = P8;
+            zs[2 * kBlock + t] = P9;
+            zs[3 * kBlock + t] = P89;
+            __syncthreads();
+            const int u = t | (1 << b);
+            if (!(t & (1 << b))) {
+                S += zs[u];
+                P8 += zs[kBlock + u];
+                P9 += zs[2 * kBlock + u];
+                P89 += zs[3 * kBlock + u];
+            }
+        }
+        __syncthreads();
+        zs[t] = S;
+        zs[kBlock + t] = P8;
+        zs[2 * kBlock + t] = P9;
+        zs[3 * kBlock + t] = P89;
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < kMomOwn; o++) {
+            if (qidx[o] >= 0 && (row & qmask[o]) == qmask[o]) acc[o].add(zs[qidx[o]]);
+        }
+    }
+    for (int o = 0; o < kMomOwn; o++) {
+        const int q = t + o * kBlock;
+        if (q < nq) part[(uint64_t)blockIdx.x * nq + q] = acc[o].value();
+    }
+}
+
+__global__ void k_moments_final(const double *__restrict__ part, int nblocks, int nq,
+                                double *__restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    Kahan k;
+    for (int b = 0; b < nblocks; b++) k.add(part[(uint64_t)b * nq + q]);
+    out[q] = k.value();
+}
+
+// ===========================================================================
+// Small helpers.
+// ===========================================================================
+template <class V>
+__global__ void k_fill(V *x, uint64_t dim, double v) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < dim; s += stride)
+        x[s] = (V)v;
+}
+
+template <class A, class B>
+__global__ void k_convert(const A *__restrict__ src, B *__restrict__ dst, uint64_t dim) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < dim; s += stride)
+        dst[s] = (B)src[s];
+}
+
+}  // namespace tasep

@@ -1,0 +1,665 @@
+// tasep_hip.hip -- C ABI (include/tasep_hip.h) over the gfx950 kernels.
+//
+// The context owns the state vector and work vectors in HBM, a run-state
+// block, per-block partial buffers and one default-priority stream.  A run
+// enqueues the step kernels in chunks; the early-exit decision is made on
+// the device by the last block of each step (finish_step), so the host only
+// reads one flag per chunk.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/tasep_hip.h"
+#include "kernels.hpp"
+
+using namespace tasep;
+
+namespace {
+
+constexpr int kMaxBlocks = 8192;       // grid cap of the streaming kernels
+constexpr int kMomMaxBlocks = 2048;
+
+thread_local char g_errmsg[256] = "";
+
+int fail(int code, const char *what, hipError_t e = hipSuccess) {
+    if (e != hipSuccess)
+        snprintf(g_errmsg, sizeof g_errmsg, "%s: %s", what, hipGetErrorString(e));
+    else
+        snprintf(g_errmsg, sizeof g_errmsg, "%s", what);
+    return code;
+}
+
+#define HIP_OK(expr)                                                        \
+    do {                                                                    \
+        hipError_t e_ = (expr);                                             \
+        if (e_ != hipSuccess)                                               \
+            return fail(e_ == hipErrorOutOfMemory ? TASEP_ERR_NOMEM         \
+                                                  : TASEP_ERR_HIP,          \
+                        #expr, e_);                                         \
+    } while (0)
+
+int check_device() {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+        return fail(TASEP_ERR_NODEV, "no HIP device visible");
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIP_OK(hipGetDeviceProperties(&p, dev));
+    if (strncmp(p.gcnArchName, "gfx950", 6) != 0)
+        return fail(TASEP_ERR_NODEV, "device is not gfx950 (MI355X)");
+    return TASEP_OK;
+}
+
+uint64_t grid_for(uint64_t dim, int cap = kMaxBlocks) {
+    uint64_t b = (dim + kBlock - 1) / kBlock;
+    return b < (uint64_t)cap ? (b ? b : 1) : (uint64_t)cap;
+}
+
+}  // namespace
+
+struct tasep_ctx {
+    int n = 0, prec = 0, integ = 0, gen = 0;
+    uint64_t dim = 0;
+    size_t esize = 8;               // bytes per state entry
+    hipStream_t stream = nullptr;
+    void *buf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int cur = 0;                    // which of buf[0], buf[1] holds x
+    double *rates = nullptr;        // 3 sets of kRateStride doubles
+    bool have_rates = false;
+    bool tv = false;
+    int32_t *tv_type = nullptr;
+    double *tv_par = nullptr;       // amp | period | offset | c, each kRateStride
+    uint64_t *rowPtr = nullptr, *colInd = nullptr;
+    double *val = nullptr;
+    bool csr_built = false;
+    RunState *st = nullptr;
+    RunState *st_host = nullptr;    // pinned
+    double *pchk = nullptr, *pmass = nullptr;
+    double *mom_part = nullptr, *mom_out = nullptr;
+    double host_rates[kRateStride] = {0};
+};
+
+// --------------------------------------------------------------- utilities
+static int ensure_buf(tasep_ctx *c, int i) {
+    if (c->buf[i]) return TASEP_OK;
+    HIP_OK(hipMalloc(&c->buf[i], c->dim * c->esize));
+    return TASEP_OK;
+}
+
+static void *xcur(tasep_ctx *c) { return c->buf[c->cur]; }
+
+template <class V>
+static void launch_fill(tasep_ctx *c, void *p, double v) {
+    hipLaunchKernelGGL(k_fill<V>, dim3(grid_for(c->dim)), dim3(kBlock), 0, c->stream,
+                       (V *)p, c->dim, v);
+}
+
+extern "C" {
+
+const char *tasep_strerror(int code) {
+    switch (code) {
+    case TASEP_OK: return "ok";
+    case TASEP_ERR_ARG: return g_errmsg[0] ? g_errmsg : "invalid argument";
+    case TASEP_ERR_NOMEM: return g_errmsg[0] ? g_errmsg : "out of memory";
+    case TASEP_ERR_HIP: return g_errmsg[0] ? g_errmsg : "HIP runtime error";
+    case TASEP_ERR_STATE: return g_errmsg[0] ? g_errmsg : "call out of order";
+    case TASEP_ERR_NODEV: return g_errmsg[0] ? g_errmsg : "no gfx950 device";
+    default: return "unknown error";
+    }
+}
+
+uint64_t tasep_csr_nnz(int n) {
+    if (n < 2 || n > 40) return 0;
+    return 2 * ((uint64_t)1 << n) + ((uint64_t)1 << (n - 2)) * (uint64_t)(n - 1);
+}
+
+int tasep_ctx_destroy(tasep_ctx *c) {
+    if (!c) return TASEP_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void *&p : c->buf) if (p) hipFree(p);
+    hipFree(c->rates);
+    hipFree(c->tv_type);
+    hipFree(c->tv_par);
+    hipFree(c->rowPtr);
+    hipFree(c->colInd);
+    hipFree(c->val);
+    hipFree(c->st);
+    if (c->st_host) hipHostFree(c->st_host);
+    hipFree(c->pchk);
+    hipFree(c->pmass);
+    hipFree(c->mom_part);
+    hipFree(c->mom_out);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return TASEP_OK;
+}
+
+int tasep_ctx_create(tasep_ctx **out, int n, int precision, int integrator, int generator) {
+    if (!out) return fail(TASEP_ERR_ARG, "ctx pointer is NULL");
+    *out = nullptr;
+    if (n < 2 || n > kMaxN) return fail(TASEP_ERR_ARG, "n must be in [2, 34]");
+    if (precision != TASEP_F64 && precision != TASEP_F32)
+        return fail(TASEP_ERR_ARG, "unknown precision");
+    if (integrator != TASEP_REF && integrator != TASEP_RK4)
+        return fail(TASEP_ERR_ARG, "unknown integrator");
+    if (generator != TASEP_MF && generator != TASEP_CSR)
+        return fail(TASEP_ERR_ARG, "unknown generator");
+    if (integrator == TASEP_REF && precision != TASEP_F64)
+        return fail(TASEP_ERR_ARG, "the reference integrator runs in FP64 only");
+    if (generator == TASEP_CSR && (precision != TASEP_F64 || n > 30))
+        return fail(TASEP_ERR_ARG, "CSR generator needs FP64 and n <= 30");
+    int rc = check_device();
+    if (rc) return rc;
+    tasep_ctx *c = new tasep_ctx;
+    c->n = n;
+    c->prec = precision;
+    c->integ = integrator;
+    c->gen = generator;
+    c->dim = (uint64_t)1 << n;
+    c->esize = precision == TASEP_F32 ? 4 : 8;
+#define CK(expr)                       \
+    do {                               \
+        int r_ = (expr);               \
+        if (r_) {                      \
+            tasep_ctx_destroy(c);      \
+            return r_;                 \
+        }                              \
+    } while (0)
+    auto hipck = [&](hipError_t e, const char *what) {
+        return e == hipSuccess ? TASEP_OK
+                               : fail(e == hipErrorOutOfMemory ? TASEP_ERR_NOMEM : TASEP_ERR_HIP, what, e);
+    };
+    CK(hipck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate"));
+    CK(hipck(hipMalloc(&c->rates, 3 * kRateStride * sizeof(double)), "hipMalloc(rates)"));
+    CK(hipck(hipMalloc(&c->st, sizeof(RunState)), "hipMalloc(state)"));
+    CK(hipck(hipHostMalloc((void **)&c->st_host, sizeof(RunState), hipHostMallocDefault),
+             "hipHostMalloc(state)"));
+    CK(hipck(hipMalloc(&c->pchk, kMaxBlocks * sizeof(double)), "hipMalloc(partials)"));
+    CK(hipck(hipMalloc(&c->pmass, kMaxBlocks * sizeof(double)), "hipMalloc(partials)"));
+    CK(ensure_buf(c, 0));
+    memset(c->st_host, 0, sizeof(RunState));
+    c->st_host->scale = 1.0;
+    CK(hipck(hipMemcpyAsync(c->st, c->st_host, sizeof(RunState), hipMemcpyHostToDevice, c->stream),
+             "hipMemcpy(state)"));
+    if (c->prec == TASEP_F32) launch_fill<float>(c, c->buf[0], 1.0 / (double)c->dim);
+    else launch_fill<double>(c, c->buf[0], 1.0 / (double)c->dim);
+    CK(hipck(hipStreamSynchronize(c->stream), "init"));
+#undef CK
+    *out = c;
+    return TASEP_OK;
+}
+
+int tasep_set_rates(tasep_ctx *c, const double *r) {
+    if (!c || !r) return fail(TASEP_ERR_ARG, "NULL argument");
+    for (int i = 0; i <= c->n; i++) {
+        if (!(r[i] >= 0.0) || !isfinite(r[i])) return fail(TASEP_ERR_ARG, "rates must be finite and >= 0");
+        c->host_rates[i] = r[i];
+    }
+    for (int s = 0; s < 3; s++)
+        HIP_OK(hipMemcpyAsync(c->rates + s * kRateStride, c->host_rates, kRateStride * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->have_rates = true;
+    c->tv = false;
+    c->csr_built = false;
+    return TASEP_OK;
+}
+
+int tasep_set_time_variant(tasep_ctx *c, const int32_t *type, const double *amp,
+                           const double *period, const double *offset, const double *cst) {
+    if (!c || !type || !amp || !period || !offset || !cst) return fail(TASEP_ERR_ARG, "NULL argument");
+    if (c->gen != TASEP_MF) return fail(TASEP_ERR_ARG, "time-variant rates need the matrix-free generator");
+    for (int i = 0; i <= c->n; i++)
+        if (type[i] < 0 || type[i] > 2) return fail(TASEP_ERR_ARG, "function type must be 0, 1 or 2");
+    if (!c->tv_type) {
+        HIP_OK(hipMalloc(&c->tv_type, kRateStride * sizeof(int32_t)));
+        HIP_OK(hipMalloc(&c->tv_par, 4 * kRateStride * sizeof(double)));
+    }
+    const size_t nb = (size_t)(c->n + 1) * sizeof(double);
+    HIP_OK(hipMemcpyAsync(c->tv_type, type, (size_t)(c->n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(c->tv_par + 0 * kRateStride, amp, nb, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(c->tv_par + 1 * kRateStride, period, nb, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(c->tv_par + 2 * kRateStride, offset, nb, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(c->tv_par + 3 * kRateStride, cst, nb, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->tv = true;
+    c->have_rates = true;
+    return TASEP_OK;
+}
+
+static int reset_scale(tasep_ctx *c) {
+    // a freshly set state is its own normalisation reference
+    c->st_host->scale = 1.0;
+    HIP_OK(hipMemcpyAsync(&c->st->scale, &c->st_host->scale, sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+    return TASEP_OK;
+}
+
+int tasep_set_x0_uniform(tasep_ctx *c) {
+    if (!c) return fail(TASEP_ERR_ARG, "NULL context");
+    c->cur = 0;
+    if (c->prec == TASEP_F32) launch_fill<float>(c, xcur(c), 1.0 / (double)c->dim);
+    else launch_fill<double>(c, xcur(c), 1.0 / (double)c->dim);
+    int rc = reset_scale(c);
+    if (rc) return rc;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return TASEP_OK;
+}
+
+int tasep_set_x0_state(tasep_ctx *c, uint64_t s) {
+    if (!c) return fail(TASEP_ERR_ARG, "NULL context");
+    if (s >= c->dim) return fail(TASEP_ERR_ARG, "state index out of range");
+    c->cur = 0;
+    HIP_OK(hipMemsetAsync(xcur(c), 0, c->dim * c->esize, c->stream));
+    if (c->prec == TASEP_F32) {
+        float one = 1.0f;
+        HIP_OK(hipMemcpyAsync((float *)xcur(c) + s, &one, 4, hipMemcpyHostToDevice, c->stream));
+    } else {
+        double one = 1.0;
+        HIP_OK(hipMemcpyAsync((double *)xcur(c) + s, &one, 8, hipMemcpyHostToDevice, c->stream));
+    }
+    int rc = reset_scale(c);
+    if (rc) return rc;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return TASEP_OK;
+}
+
+int tasep_set_x(tasep_ctx *c, const double *x) {
+    if (!c || !x) return fail(TASEP_ERR_ARG, "NULL argument");
+    c->cur = 0;
+    if (c->prec == TASEP_F64) {
+        HIP_OK(hipMemcpyAsync(xcur(c), x, c->dim * 8, hipMemcpyHostToDevice, c->stream));
+    } else {
+        double *tmp = nullptr;
+        HIP_OK(hipMallocAsync((void **)&tmp, c->dim * 8, c->stream));
+        HIP_OK(hipMemcpyAsync(tmp, x, c->dim * 8, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(c->dim)), dim3(kBlock), 0, c->stream,
+                           tmp, (float *)xcur(c), c->dim);
+        HIP_OK(hipFreeAsync(tmp, c->stream));
+    }
+    int rc = reset_scale(c);
+    if (rc) return rc;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return TASEP_OK;
+}
+
+int tasep_get_x(tasep_ctx *c, double *x) {
+    if (!c || !x) return fail(TASEP_ERR_ARG, "NULL argument");
+    if (c->prec == TASEP_F64) {
+        HIP_OK(hipMemcpyAsync(x, xcur(c), c->dim * 8, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        double *tmp = nullptr;
+        HIP_OK(hipMallocAsync((void **)&tmp, c->dim * 8, c->stream));
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(c->dim)), dim3(kBlock), 0, c->stream,
+                           (const float *)xcur(c), tmp, c->dim);
+        HIP_OK(hipMemcpyAsync(x, tmp, c->dim * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipFreeAsync(tmp, c->stream));
+    }
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return TASEP_OK;
+}
+
+int tasep_build_csr(tasep_ctx *c) {
+    if (!c) return fail(TASEP_ERR_ARG, "NULL context");
+    if (c->gen != TASEP_CSR) return fail(TASEP_ERR_ARG, "context was not created with TASEP_CSR");
+    if (!c->have_rates || c->tv) return fail(TASEP_ERR_STATE, "set constant rates before building the CSR");
+    const uint64_t nnz = tasep_csr_nnz(c->n);
+    if (!c->rowPtr) {
+        HIP_OK(hipMalloc(&c->rowPtr, (c->dim + 1) * sizeof(uint64_t)));
+        HIP_OK(hipMalloc(&c->colInd, nnz * sizeof(uint64_t)));
+        HIP_OK(hipMalloc(&c->val, nnz * sizeof(double)));
+    }
+    hipLaunchKernelGGL(k_csr_build, dim3(grid_for(c->dim, 1 << 16)), dim3(kBlock), 0, c->stream, c->n,
+                       c->rates, c->rowPtr, c->colInd, c->val, c->dim);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->csr_built = true;
+    return TASEP_OK;
+}
+
+int tasep_get_csr(tasep_ctx *c, uint64_t *rowPtr, uint64_t *colInd, double *val) {
+    if (!c || !rowPtr || !colInd || !val) return fail(TASEP_ERR_ARG, "NULL argument");
+    if (!c->csr_built) return fail(TASEP_ERR_STATE, "CSR not built");
+    const uint64_t nnz = tasep_csr_nnz(c->n);
+    HIP_OK(hipMemcpyAsync(rowPtr, c->rowPtr, (c->dim + 1) * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(colInd, c->colInd, nnz * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(val, c->val, nnz * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return TASEP_OK;
+}
+
+int tasep_gencrs(uint64_t *rowPtr, double *valArr, uint64_t *colInd, const double *r, uint64_t dim,
+                 int n) {
+    if (!rowPtr || !valArr || !colInd || !r) return fail(TASEP_ERR_ARG, "NULL argument");
+    if (n < 2 || n > 30 || dim != ((uint64_t)1 << n)) return fail(TASEP_ERR_ARG, "dim must be 2^n, 2 <= n <= 30");
+    tasep_ctx *c = nullptr;
+    int rc = tasep_ctx_create(&c, n, TASEP_F64, TASEP_REF, TASEP_CSR);
+    if (rc) return rc;
+    if (!(rc = tasep_set_rates(c, r)) && !(rc = tasep_build_csr(c)))
+        rc = tasep_get_csr(c, rowPtr, colInd, valArr);
+    tasep_ctx_destroy(c);
+    return rc;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- the loop
+template <class V>
+static PassArgs<V> pass_base(tasep_ctx *c) {
+    PassArgs<V> a;
+    memset(&a, 0, sizeof a);
+    a.rates = c->rates;
+    a.rowPtr = c->rowPtr;
+    a.colInd = c->colInd;
+    a.val = c->val;
+    a.st = c->st;
+    a.pchk = c->pchk;
+    a.pmass = c->pmass;
+    a.dim = c->dim;
+    a.g = make_lattice(c->n);
+    return a;
+}
+
+// Optional per-launch timing (tasep_step_profile): events around each launch.
+struct LaunchProf {
+    bool on = false;
+    int count = 0;
+    hipEvent_t ev[2 * 8];
+    const char *name[8];
+    double bytes[8];
+};
+static thread_local LaunchProf *g_prof = nullptr;
+
+static void prof_begin(tasep_ctx *c) {
+    if (g_prof && g_prof->count < 8) hipEventRecord(g_prof->ev[2 * g_prof->count], c->stream);
+}
+static void prof_end(tasep_ctx *c, const char *name, double bytes_per_state) {
+    if (g_prof && g_prof->count < 8) {
+        hipEventRecord(g_prof->ev[2 * g_prof->count + 1], c->stream);
+        g_prof->name[g_prof->count] = name;
+        g_prof->bytes[g_prof->count] = bytes_per_state;
+        g_prof->count++;
+    }
+}
+
+template <class V, bool CSR, bool TWO, int EPI, bool SELF = false>
+static void launch_pass(tasep_ctx *c, const PassArgs<V> &a, const char *name, double bytes) {
+    prof_begin(c);
+    hipLaunchKernelGGL((k_rk_pass<V, CSR, TWO, EPI, SELF>), dim3(grid_for(c->dim)), dim3(kBlock), 0,
+                       c->stream, a);
+    prof_end(c, name, bytes);
+}
+
+// One classical RK4 step in Horner form (constant rates; x updated in place).
+template <class V, bool CSR>
+static void rk4_horner_step(tasep_ctx *c, double h) {
+    V *x = (V *)xcur(c);
+    V *u = (V *)c->buf[2];
+    V *v = (V *)c->buf[3];
+    PassArgs<V> a = pass_base<V>(c);
+    a.x = x;
+    a.use_self = 1;
+    const double e = (double)sizeof(V);
+    const double csr = CSR ? 8.0 + 16.0 * (2.0 + (c->n - 1) / 4.0) : 0.0;
+    // P1: u = x + h/4 L x          (read x, write u)
+    a.gx = x; a.gx_is_x = 1; a.out = u; a.c = 0.25 * h;
+    launch_pass<V, CSR, false, EPI_STORE, true>(c, a, "horner_p1", 2 * e + csr);
+    // P2: v = x + h/3 L u          (read u, x, write v)
+    a.gx = u; a.gx_is_x = 0; a.out = v; a.c = h / 3.0;
+    launch_pass<V, CSR, false, EPI_STORE>(c, a, "horner_p23", 3 * e + csr);
+    // P3: u = x + h/2 L v
+    a.gx = v; a.out = u; a.c = 0.5 * h;
+    launch_pass<V, CSR, false, EPI_STORE>(c, a, "horner_p23", 3 * e + csr);
+    // P4: x = x + h L u            (read u, x, write x)
+    a.gx = u; a.out = x; a.c = h;
+    launch_pass<V, CSR, false, EPI_HORNER>(c, a, "horner_p4", 3 * e + csr);
+}
+
+// One classical RK4 step, staged (time-variant rates): x -> xn (ping-pong).
+template <class V, bool CSR>
+static void rk4_staged_step(tasep_ctx *c, double h, uint64_t j) {
+    V *x = (V *)c->buf[c->cur ^ (int)(j & 1)];
+    V *xn = (V *)c->buf[c->cur ^ (int)((j + 1) & 1)];
+    V *k1 = (V *)c->buf[2], *k2 = (V *)c->buf[3], *k3 = (V *)c->buf[4];
+    PassArgs<V> a = pass_base<V>(c);
+    a.x = x;
+    a.gx = x;
+    a.gx_is_x = 1;
+    a.c = h;
+    a.use_self = 0;
+    const double e = (double)sizeof(V);
+    a.rset = 0; a.out = k1;
+    launch_pass<V, CSR, false, EPI_STORE>(c, a, "rk4_s1", 2 * e);
+    a.rset = 1; a.gk = k1; a.ck = 0.5; a.out = k2;
+    launch_pass<V, CSR, true, EPI_STORE>(c, a, "rk4_s23", 3 * e);
+    a.rset = 1; a.gk = k2; a.ck = 0.5; a.out = k3;
+    launch_pass<V, CSR, true, EPI_STORE>(c, a, "rk4_s23", 3 * e);
+    a.rset = 2; a.gk = k3; a.ck = 1.0; a.out = xn; a.k1 = k1; a.k2 = k2; a.k3 = k3;
+    launch_pass<V, CSR, true, EPI_RK4>(c, a, "rk4_s4", 5 * e);
+}
+
+static void ref_step(tasep_ctx *c, double h, uint64_t j) {
+    RefArgs a;
+    a.x = (const double *)c->buf[c->cur ^ (int)(j & 1)];
+    a.xn = (double *)c->buf[c->cur ^ (int)((j + 1) & 1)];
+    a.rates = c->rates;
+    a.rowPtr = c->rowPtr;
+    a.colInd = c->colInd;
+    a.val = c->val;
+    a.st = c->st;
+    a.pchk = c->pchk;
+    a.pmass = c->pmass;
+    a.dim = c->dim;
+    a.h = h;
+    a.g = make_lattice(c->n);
+    prof_begin(c);
+    if (c->gen == TASEP_CSR) {
+        hipLaunchKernelGGL(k_ref_step<true>, dim3(grid_for(c->dim)), dim3(kBlock), 0, c->stream, a);
+        prof_end(c, "ref_step_csr", 24.0 + 16.0 * (2.0 + (c->n - 1) / 4.0));
+    } else {
+        hipLaunchKernelGGL(k_ref_step<false>, dim3(grid_for(c->dim)), dim3(kBlock), 0, c->stream, a);
+        prof_end(c, "ref_step_mf", 16.0);
+    }
+}
+
+// Allocate what a run needs and reset the device loop state.
+static int prepare_run(tasep_ctx *c, double t0, double h, double thr, int patience) {
+    if (!c->have_rates) return fail(TASEP_ERR_STATE, "set rates before running");
+    if (!isfinite(h) || !isfinite(t0)) return fail(TASEP_ERR_ARG, "t0 and h must be finite");
+    if (c->gen == TASEP_CSR && !c->csr_built) {
+        int rc = tasep_build_csr(c);
+        if (rc) return rc;
+    }
+    const bool staged = c->integ == TASEP_RK4 && c->tv;
+    if (c->integ == TASEP_REF || staged) {
+        int rc = ensure_buf(c, 1);
+        if (rc) return rc;
+    }
+    if (c->integ == TASEP_RK4) {
+        for (int i = 2; i < (staged ? 5 : 4); i++) {
+            int rc = ensure_buf(c, i);
+            if (rc) return rc;
+        }
+    }
+    // fresh loop state; an FP32 vector keeps the scale of its stored mass
+    HIP_OK(hipMemcpyAsync(c->st_host, c->st, sizeof(RunState), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    const double scale = c->st_host->scale;
+    memset(c->st_host, 0, sizeof(RunState));
+    c->st_host->t = t0;
+    c->st_host->h = h;
+    c->st_host->thr_h = thr * h;
+    c->st_host->patience = c->tv ? -1 : patience;
+    c->st_host->renorm = c->prec == TASEP_F32 ? 1 : 0;
+    c->st_host->scale = c->prec == TASEP_F32 ? scale : 1.0;
+    HIP_OK(hipMemcpyAsync(c->st, c->st_host, sizeof(RunState), hipMemcpyHostToDevice, c->stream));
+    return TASEP_OK;
+}
+
+// Enqueue step j of the current run.
+static void enqueue_step(tasep_ctx *c, double h, uint64_t j) {
+    const bool staged = c->integ == TASEP_RK4 && c->tv;
+    if (c->tv) {
+        TvArgs tva;
+        tva.type = c->tv_type;
+        tva.amp = c->tv_par;
+        tva.period = c->tv_par + kRateStride;
+        tva.offset = c->tv_par + 2 * kRateStride;
+        tva.c = c->tv_par + 3 * kRateStride;
+        hipLaunchKernelGGL(k_tv_rates, dim3(1), dim3(64), 0, c->stream, c->st, tva, c->rates, c->n,
+                           c->integ == TASEP_RK4 ? 3 : 1);
+    }
+    if (c->integ == TASEP_REF) {
+        ref_step(c, h, j);
+    } else if (c->prec == TASEP_F32) {
+        if (staged) rk4_staged_step<float, false>(c, h, j);
+        else rk4_horner_step<float, false>(c, h);
+    } else if (c->gen == TASEP_CSR) {
+        rk4_horner_step<double, true>(c, h);
+    } else {
+        if (staged) rk4_staged_step<double, false>(c, h, j);
+        else rk4_horner_step<double, false>(c, h);
+    }
+}
+
+// Read the loop state back and move the ping-pong index.
+static int finish_run(tasep_ctx *c, tasep_run_info *info) {
+    HIP_OK(hipMemcpyAsync(c->st_host, c->st, sizeof(RunState), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    const RunState &s = *c->st_host;
+    const bool staged = c->integ == TASEP_RK4 && c->tv;
+    if (c->integ == TASEP_REF || staged) c->cur ^= (int)(s.steps_done & 1);
+    if (info) {
+        info->steps_done = s.steps_done;
+        info->steps_at_stop = s.steps_at_stop;
+        info->early_stop = s.early_stop;
+        info->stopped = s.stopped;
+        info->t = s.t;
+        info->last_checksum = s.checksum;
+        info->mass = s.mass;
+    }
+    return TASEP_OK;
+}
+
+extern "C" {
+
+int tasep_run(tasep_ctx *c, double t0, double h, uint64_t steps, double thr, int patience,
+              tasep_run_info *info) {
+    if (!c) return fail(TASEP_ERR_ARG, "NULL context");
+    int rc = prepare_run(c, t0, h, thr, patience);
+    if (rc) return rc;
+    const bool watch = c->st_host->patience >= 0;
+    // steps per host check of the exit flag: about 2^26 state-updates
+    uint64_t chunk = ((uint64_t)1 << 26) >> c->n;
+    if (chunk < 4) chunk = 4;
+    if (chunk > 4096) chunk = 4096;
+    uint64_t j = 0;
+    while (j < steps) {
+        const uint64_t end = j + chunk < steps ? j + chunk : steps;
+        for (; j < end; j++) enqueue_step(c, h, j);
+        HIP_OK(hipGetLastError());
+        if (watch && j < steps) {
+            HIP_OK(hipMemcpyAsync(&c->st_host->stopped, &c->st->stopped, sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIP_OK(hipStreamSynchronize(c->stream));
+            if (c->st_host->stopped) break;
+        }
+    }
+    return finish_run(c, info);
+}
+
+int tasep_step_profile(tasep_ctx *c, double t0, double h, int reps, tasep_kernel_time *out, int max,
+                       int *count) {
+    if (!c || !out || !count || reps < 1 || max < 1) return fail(TASEP_ERR_ARG, "bad argument");
+    int rc = prepare_run(c, t0, h, 0.0, -1);
+    if (rc) return rc;
+    LaunchProf prof;
+    for (int i = 0; i < 16; i++) HIP_OK(hipEventCreate(&prof.ev[i]));
+    std::vector<double> acc(8, 0.0);
+    int nk = 0;
+    for (int r = 0; r < reps; r++) {
+        prof.count = 0;
+        g_prof = &prof;
+        enqueue_step(c, h, (uint64_t)r);
+        g_prof = nullptr;
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(c->stream));
+        nk = prof.count;
+        for (int k = 0; k < nk; k++) {
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, prof.ev[2 * k], prof.ev[2 * k + 1]));
+            acc[k] += ms;
+        }
+    }
+    for (int i = 0; i < 16; i++) hipEventDestroy(prof.ev[i]);
+    *count = nk < max ? nk : max;
+    for (int k = 0; k < *count; k++) {
+        snprintf(out[k].name, sizeof out[k].name, "%s", prof.name[k]);
+        out[k].ms = acc[k] / reps;
+        out[k].bytes_per_state = prof.bytes[k];
+    }
+    return finish_run(c, nullptr);
+}
+
+int tasep_moments(tasep_ctx *c, double *y, double *cov, double *mul, double *mass) {
+    if (!c) return fail(TASEP_ERR_ARG, "NULL context");
+    const int n = c->n;
+    const int nq = n * (n + 1) / 2 + 1;
+    const uint64_t rows = c->dim >> kMomRowBits ? c->dim >> kMomRowBits : 1;
+    const int nb = (int)(rows < (uint64_t)kMomMaxBlocks ? rows : kMomMaxBlocks);
+    if (!c->mom_part) {
+        HIP_OK(hipMalloc(&c->mom_part, (size_t)kMomMaxBlocks * kMomMaxQ * sizeof(double)));
+        HIP_OK(hipMalloc(&c->mom_out, kMomMaxQ * sizeof(double)));
+    }
+    if (c->prec == TASEP_F32)
+        hipLaunchKernelGGL(k_moments_partial<float>, dim3(nb), dim3(kBlock), 0, c->stream,
+                           (const float *)xcur(c), n, c->dim, rows, c->mom_part);
+    else
+        hipLaunchKernelGGL(k_moments_partial<double>, dim3(nb), dim3(kBlock), 0, c->stream,
+                           (const double *)xcur(c), n, c->dim, rows, c->mom_part);
+    hipLaunchKernelGGL(k_moments_final, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
+                       c->mom_part, nb, nq, c->mom_out);
+    HIP_OK(hipGetLastError());
+    std::vector<double> E(nq);
+    HIP_OK(hipMemcpyAsync(E.data(), c->mom_out, nq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    double M = E[nq - 1];
+    if (c->prec == TASEP_F32 && M > 0.0) {
+        for (int q = 0; q < nq - 1; q++) E[q] /= M;
+        M = 1.0;
+    }
+    if (mass) *mass = M;
+    std::vector<double> yy(n);
+    for (int i = 0; i < n; i++) yy[i] = E[tri_index(i, i)];
+    if (y) for (int i = 0; i < n; i++) y[i] = yy[i];
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < n; k++) {
+            const double e = E[i <= k ? tri_index(i, k) : tri_index(k, i)];
+            // sum_s (b_k - y_k)(b_i - y_i) x_s = E_ik - y_i y_k (2 - M)
+            const double cv = e - yy[i] * yy[k] * (2.0 - M);
+            if (cov) cov[i * n + k] = cv;
+            if (mul) mul[i * n + k] = yy[k] * yy[i] + cv;
+        }
+    }
+    return TASEP_OK;
+}
+
+int tasep_get_stream(tasep_ctx *c, void **stream) {
+    if (!c || !stream) return fail(TASEP_ERR_ARG, "NULL argument");
+    *stream = (void *)c->stream;
+    return TASEP_OK;
+}
+
+int tasep_get_x_device(tasep_ctx *c, void **ptr) {
+    if (!c || !ptr) return fail(TASEP_ERR_ARG, "NULL argument");
+    *ptr = xcur(c);
+    return TASEP_OK;
+}
+
+}  // extern "C"

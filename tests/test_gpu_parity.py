@@ -1,0 +1,251 @@
+"""Parity of the HIP path (through the C ABI) with the reference and the oracle.
+
+Bars: bit-exact for the CSR arrays and for the FP64 reference-faithful loop
+(TASEP_REF) with constant rates; <= 1e-12 where a libm function differs
+(time-variant rates: device sin vs glibc sin) and for moments (parallel
+FP64 sums vs the reference's sequential ones); stated tolerances for RK4.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import tasep_amd as T
+from tests import _golden as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits_equal(a, b):
+    return np.array_equal(np.asarray(a, np.float64).view(np.uint64),
+                          np.asarray(b, np.float64).view(np.uint64))
+
+
+# ------------------------------------------------------------------ CSR (a1)
+@pytest.mark.parametrize("fx", G.csr_fixtures(), ids=lambda f: f["name"])
+def test_device_csr_bit_exact_vs_reference_fixture(fx):
+    rp, ci, va = T.gencrs(fx["n"], fx["rates"])
+    assert np.array_equal(rp, fx["rowPtr"])
+    assert np.array_equal(ci, fx["colInd"])
+    assert _bits_equal(va, fx["val"])
+
+
+@pytest.mark.parametrize("n", [2, 3, 13, 17])
+def test_device_csr_bit_exact_vs_oracle(n):
+    r = np.random.default_rng(100 + n).uniform(0.1, 2.0, n + 1)
+    rp, ci, va = T.gencrs(n, r)
+    orp, oci, ova = oracle.gencrs(n, r)
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci) and _bits_equal(va, ova)
+    assert int(rp[-1]) == T.csr_nnz(n)
+
+
+# ---------------------------------------------------- reference loop (a2-a6)
+@pytest.mark.parametrize("gen", ["mf", "csr"])
+@pytest.mark.parametrize("name", G.cli_case_names())
+def test_ref_loop_vs_reference_cli_outputs(name, gen):
+    c = G.cli_case(name)
+    if c["tv"] is not None and gen == "csr":
+        pytest.skip("the reference refuses -f with --time-var")
+    n = c["n"]
+    h = oracle.step_h(c["T"], c["t0"], c["steps"])
+    with T.Engine(n, "f64", "ref", gen) as e:
+        if c["tv"] is not None:
+            e.set_time_variant(c["tv"])
+        else:
+            e.set_rates(c["rates"])
+        e.set_x0_uniform()
+        info = e.run(c["t0"], h, c["steps"], thr=1e-7, patience=100)
+        x = e.get_x()
+        y, cov, mul, mass = e.moments()
+    if c["tv"] is None:
+        assert _bits_equal(x, c["x"]), f"max|dx| = {np.max(np.abs(x - c['x']))}"
+    else:
+        # device sin/cos vs glibc: not bit-exact, bounded
+        assert np.max(np.abs(x - c["x"])) <= 1e-12
+    conv = G.stdout_value(c, "Convergence after ")
+    if conv is not None:
+        assert info.stopped and int(conv.split()[0]) == info.steps_at_stop
+    else:
+        assert not info.stopped and info.steps_done == c["steps"]
+    assert G.stdout_value(c, "Simulation time elapsed:") == "%f" % info.t
+    assert G.stdout_value(c, "Checksum over result vector:") == "%f" % mass
+    # moments vs the reference's written files (entry site first)
+    vec = ", ".join("%f" % y[n - 1 - i] for i in range(n))
+    assert G.read_text(c, "out_exp_values_fin.vec") == vec
+    ocov, omul = oracle.cov(n, c["x"])
+    assert np.max(np.abs(cov[::-1, ::-1] - ocov)) <= 1e-12
+    assert np.max(np.abs(mul[::-1, ::-1] - omul)) <= 1e-12
+
+
+@pytest.mark.parametrize("gen", ["mf", "csr"])
+@pytest.mark.parametrize("n", [5, 12, 16])
+def test_ref_loop_bit_exact_vs_oracle(n, gen):
+    r = np.random.default_rng(n).uniform(0.1, 2.0, n + 1)
+    x0 = np.random.default_rng(n + 1).random(1 << n)
+    x0 /= x0.sum()
+    steps = 40
+    ox, oinfo = oracle.run_ref(n, r, x0, 0.0, 0.01, steps, patience=-1)
+    with T.Engine(n, "f64", "ref", gen) as e:
+        e.set_rates(r)
+        e.set_x(x0)
+        info = e.run(0.0, 0.01, steps, patience=-1)
+        x = e.get_x()
+    assert info.steps_done == steps and oinfo.steps_done == steps
+    assert _bits_equal(x, ox), f"max|dx| = {np.max(np.abs(x - ox))}"
+    assert info.t == oinfo.t
+    assert abs(info.last_checksum - oinfo.last_checksum) <= 1e-14 * oinfo.last_checksum
+
+
+def test_ref_early_stop_config1_uniform_is_stationary():
+    """n=10, alpha=beta=1/2, gamma=1 from the uniform state: the reference
+    leaves after 101 sub-threshold steps (index 100, t = 1.01)."""
+    n = 10
+    r = np.array([0.5] + [1.0] * (n - 1) + [0.5])
+    with T.Engine(n, "f64", "ref", "csr") as e:
+        e.set_rates(r)
+        e.set_x0_uniform()
+        info = e.run(0.0, oracle.step_h(100.0, 0.0, 10000), 10000)
+        x = e.get_x()
+    assert info.stopped and info.steps_at_stop == 100 and info.steps_done == 101
+    assert "%f" % info.t == "1.010000"
+    assert np.all(x == 2.0 ** -10)
+
+
+def test_ref_config1_from_empty_lattice_vs_oracle():
+    """Configuration 1 proper: p[0] = 1 (empty lattice), t = 0..100."""
+    n = 10
+    r = np.array([0.5] + [1.0] * (n - 1) + [0.5])
+    h = oracle.step_h(100.0, 0.0, 10000)
+    x0 = np.zeros(1 << n)
+    x0[0] = 1.0
+    ox, oinfo = oracle.run_ref(n, r, x0, 0.0, h, 10000, use_csr=True)
+    for gen in ("csr", "mf"):
+        with T.Engine(n, "f64", "ref", gen) as e:
+            e.set_rates(r)
+            e.set_x0_state(0)
+            info = e.run(0.0, h, 10000)
+            x = e.get_x()
+            y, cov, _, _ = e.moments()
+        assert _bits_equal(x, ox)
+        assert info.steps_done == oinfo.steps_done and info.stopped == oinfo.stopped
+        assert np.max(np.abs(y - oracle.site_moments(n, ox))) <= 1e-12
+
+
+# ------------------------------------------------------------ moments (a7-a9)
+@pytest.mark.parametrize("n", [2, 7, 10, 11, 15, 18])
+def test_moments_vs_oracle(n):
+    x = np.random.default_rng(7 * n).random(1 << n)
+    x /= x.sum()
+    with T.Engine(n, "f64", "ref", "mf") as e:
+        e.set_x(x)
+        y, cov, mul, mass = e.moments()
+    oy = oracle.site_moments(n, x)
+    ocov, omul = oracle.cov(n, x, oy)
+    assert np.max(np.abs(y - oy)) <= 1e-13
+    assert abs(mass - oracle.mass(x)) <= 1e-13
+    assert np.max(np.abs(cov[::-1, ::-1] - ocov)) <= 1e-13
+    assert np.max(np.abs(mul[::-1, ::-1] - omul)) <= 1e-13
+
+
+# ------------------------------------------------------------------- RK4 mode
+@pytest.mark.parametrize("gen", ["mf", "csr"])
+def test_rk4_f64_vs_oracle(gen):
+    n = 11
+    r = np.random.default_rng(3).uniform(0.2, 1.5, n + 1)
+    x0 = np.zeros(1 << n)
+    x0[0] = 1.0
+    ox, _ = oracle.run_rk4(n, r, x0, 0.0, 0.01, 200)
+    with T.Engine(n, "f64", "rk4", gen) as e:
+        e.set_rates(r)
+        e.set_x0_state(0)
+        info = e.run(0.0, 0.01, 200, patience=-1)
+        x = e.get_x()
+    assert info.steps_done == 200
+    assert np.max(np.abs(x - ox)) <= 1e-12
+
+
+def test_rk4_f64_time_variant_vs_oracle():
+    n = 9
+    rng = np.random.default_rng(4)
+    tv = dict(type=np.array([0, 1, 2] * 3 + [0], np.int32)[: n + 1],
+              amp=rng.uniform(0, 0.5, n + 1), period=rng.uniform(0.5, 2, n + 1),
+              offset=rng.uniform(0, 1, n + 1), c=np.ones(n + 1))
+    x0 = oracle.uniform_x0(n)
+    ox, _ = oracle.run_rk4(n, None, x0, 0.0, 0.01, 150, tv=tv)
+    with T.Engine(n, "f64", "rk4", "mf") as e:
+        e.set_time_variant(tv)
+        e.set_x0_uniform()
+        info = e.run(0.0, 0.01, 150)
+        x = e.get_x()
+    assert info.steps_done == 150
+    assert np.max(np.abs(x - ox)) <= 1e-12
+
+
+def test_rk4_f32_densities_vs_f64_oracle():
+    n = 14
+    rng = np.random.default_rng(28)
+    r = np.concatenate([[rng.uniform(0.2, 1.0)], rng.lognormal(0, 0.3, n - 1),
+                        [rng.uniform(0.2, 1.0)]])
+    x0 = oracle.uniform_x0(n)
+    ox, _ = oracle.run_rk4(n, r, x0, 0.0, 0.01, 500)
+    oy = oracle.site_moments(n, ox)
+    ocov, _ = oracle.cov(n, ox, oy)
+    with T.Engine(n, "f32", "rk4", "mf") as e:
+        e.set_rates(r)
+        e.set_x0_uniform()
+        e.run(0.0, 0.01, 500)
+        y, cov, _, mass = e.moments()
+    assert np.max(np.abs(y - oy)) <= 1e-6
+    assert np.max(np.abs(cov[::-1, ::-1] - ocov)) <= 1e-6
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_rk4_bernoulli_product_is_stationary_large(prec):
+    """alpha + beta = 1, gamma = 1: the Bernoulli(alpha) product measure is
+    the exact stationary state, at any n (a size-independent property)."""
+    n = 22
+    a = 0.25
+    r = np.array([a] + [1.0] * (n - 1) + [1 - a])
+    s = np.arange(1 << n, dtype=np.uint64)
+    occ = np.zeros(1 << n)
+    for k in range(n):
+        occ += ((s >> np.uint64(k)) & np.uint64(1)).astype(np.float64)
+    x0 = a ** occ * (1 - a) ** (n - occ)
+    with T.Engine(n, prec, "rk4", "mf") as e:
+        e.set_rates(r)
+        e.set_x(x0)
+        info = e.run(0.0, 0.01, 20, patience=-1)
+        y, cov, _, mass = e.moments()
+    tol = 1e-6 if prec == "f32" else 1e-12
+    assert np.max(np.abs(y - a)) <= tol
+    assert np.max(np.abs(cov - np.diag(np.diag(cov)))) <= tol
+    assert info.last_checksum <= (1e-4 if prec == "f32" else 1e-12)
+
+
+def test_converged_rk4_matches_dehp():
+    from oracle import dehp
+    n = 12
+    a, b = 0.35, 0.6
+    r = np.array([a] + [1.0] * (n - 1) + [b])
+    with T.Engine(n, "f64", "rk4", "mf") as e:
+        e.set_rates(r)
+        e.set_x0_uniform()
+        info = e.run(0.0, 0.05, 40000, thr=1e-10, patience=100)
+        y, _, _, _ = e.moments()
+    assert info.stopped
+    rho = np.array([float(v) for v in dehp.densities(n, a, b)])
+    assert np.max(np.abs(y[::-1] - rho)) < 1e-7
+
+
+# ------------------------------------------------------------ error handling
+def test_errors_are_reported():
+    with pytest.raises(T.TasepError):
+        T.Engine(40)
+    with pytest.raises(T.TasepError):
+        T.Engine(10, "f32", "ref", "mf")
+    with T.Engine(6, "f64", "ref", "mf") as e:
+        with pytest.raises(T.TasepError):
+            e.run(0.0, 0.01, 10)          # no rates yet
+        with pytest.raises(T.TasepError):
+            e.set_rates(np.array([0.5, -1, 1, 1, 1, 1, 0.5]))
+        with pytest.raises(T.TasepError):
+            e.set_x0_state(1 << 6)
