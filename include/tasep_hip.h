@@ -112,6 +112,15 @@ int tasep_gencrs(uint64_t *rowPtr, double *valArr, uint64_t *colInd,
 int tasep_run(tasep_ctx *ctx, double t0, double h, uint64_t steps, double thr,
               int patience, tasep_run_info *info);
 
+/* tasep_run in pieces: with resume != 0 the loop continues where the last
+ * call stopped (t, the step index and the early-exit counters are kept;
+ * t0 is ignored), so a caller can take snapshots every few steps as the
+ * reference's -a option does inside its loop (src/masterEquation.c:646-666).
+ * info->steps_done counts all steps since the last call with resume == 0. */
+int tasep_run_resume(tasep_ctx *ctx, double t0, double h, uint64_t steps,
+                     double thr, int patience, int resume,
+                     tasep_run_info *info);
+
 /* Moments of the current state in one device pass with FP64 sums
  * (replaces condense/exp_val_eval, src/lib/matvec.c:10-91, and the two
  * O(n^2 2^n) loops at src/masterEquation.c:1127-1139,1291-1303):
@@ -138,6 +147,22 @@ typedef struct {
 } tasep_kernel_time;
 int tasep_step_profile(tasep_ctx *ctx, double t0, double h, int reps,
                        tasep_kernel_time *out, int max, int *count);
+
+/* Engine options (no reference counterpart; defaults in brackets):
+ *   TASEP_OPT_TILED        [1] tiled matrix-free passes when n is large
+ *                              enough (n >= 12 FP32, n >= 11 FP64); 0 = one
+ *                              lane per state everywhere
+ *   TASEP_OPT_XCD_ORDER    [1] tiles walked in contiguous eighths per XCD
+ *   TASEP_OPT_RK4_STAGED   [0] RK4 with four stored stages even for constant
+ *                              rates (default: the Horner form of the same
+ *                              step, 3 work vectors instead of 5)
+ *   TASEP_OPT_TILE_BLOCKS  [0] workgroups of the tiled passes (multiple of
+ *                              8); 0 = as many as are resident at once */
+#define TASEP_OPT_TILED        1
+#define TASEP_OPT_XCD_ORDER    2
+#define TASEP_OPT_RK4_STAGED   3
+#define TASEP_OPT_TILE_BLOCKS  4
+int tasep_set_option(tasep_ctx *ctx, int key, int64_t value);
 
 /* The context's HIP stream (hipStream_t) and device pointer of the current
  * state vector (double* or float* by precision), for callers that time or

@@ -30,8 +30,9 @@ EXPORTS = [
     "tasep_set_x", "tasep_get_x", "tasep_build_csr", "tasep_get_csr",
     "tasep_csr_nnz", "tasep_gencrs", "tasep_run", "tasep_moments",
     "tasep_get_stream", "tasep_get_x_device", "tasep_strerror",
-    "tasep_step_profile",
+    "tasep_step_profile", "tasep_set_option", "tasep_run_resume",
 ]
+OPT_TILED, OPT_XCD_ORDER, OPT_RK4_STAGED, OPT_TILE_BLOCKS = 1, 2, 3, 4
 
 
 class TasepError(RuntimeError):
@@ -96,10 +97,13 @@ def load_library():
     L.tasep_gencrs.argtypes = [_u64p, _f64p, _u64p, _f64p, C.c_uint64, C.c_int]
     L.tasep_run.argtypes = [vp, C.c_double, C.c_double, C.c_uint64, C.c_double,
                             C.c_int, C.POINTER(_RunInfo)]
+    L.tasep_run_resume.argtypes = [vp, C.c_double, C.c_double, C.c_uint64, C.c_double,
+                                   C.c_int, C.c_int, C.POINTER(_RunInfo)]
     L.tasep_moments.argtypes = [vp, vp, vp, vp, vp]
     L.tasep_step_profile.argtypes = [vp, C.c_double, C.c_double, C.c_int,
                                      C.POINTER(_KernelTime), C.c_int,
                                      C.POINTER(C.c_int)]
+    L.tasep_set_option.argtypes = [vp, C.c_int, C.c_int64]
     L.tasep_get_stream.argtypes = [vp, C.POINTER(vp)]
     L.tasep_get_x_device.argtypes = [vp, C.POINTER(vp)]
     for name in EXPORTS:
@@ -193,6 +197,9 @@ class Engine:
         _check(load_library().tasep_get_x(self._h, x))
         return x
 
+    def set_option(self, key: int, value: int):
+        _check(load_library().tasep_set_option(self._h, key, value))
+
     # -- generator
     def build_csr(self):
         _check(load_library().tasep_build_csr(self._h))
@@ -206,9 +213,10 @@ class Engine:
         return rp, ci, va
 
     # -- loop
-    def run(self, t0, h, steps, thr=1e-7, patience=100) -> RunInfo:
+    def run(self, t0, h, steps, thr=1e-7, patience=100, resume=False) -> RunInfo:
         ri = _RunInfo()
-        _check(load_library().tasep_run(self._h, t0, h, steps, thr, patience, C.byref(ri)))
+        _check(load_library().tasep_run_resume(self._h, t0, h, steps, thr, patience,
+                                               1 if resume else 0, C.byref(ri)))
         return RunInfo(int(ri.steps_done), int(ri.steps_at_stop), int(ri.early_stop),
                        bool(ri.stopped), float(ri.t), float(ri.last_checksum),
                        float(ri.mass))

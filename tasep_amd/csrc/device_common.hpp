@@ -52,8 +52,10 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Fixed-shape block sum (deterministic for a fixed launch shape).  Result is
-// valid in every thread.  `red` must hold kWaves doubles.
+// Fixed-shape block sum (deterministic for a fixed launch shape) over a
+// block of NT lanes.  Result is valid in every thread.  `red` must hold
+// NT/64 doubles.
+template <int NT = kBlock>
 __device__ __forceinline__ double block_sum(double v, double *red) {
     v = wave_sum(v);
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
@@ -62,7 +64,7 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
     __syncthreads();
     double r = 0.0;
 #pragma unroll
-    for (int i = 0; i < kWaves; i++) r += red[i];
+    for (int i = 0; i < NT / kWave; i++) r += red[i];
     return r;
 }
 
@@ -76,10 +78,11 @@ __device__ __forceinline__ double load_relaxed(const double *p) {
 // which block is last) and advances the run state exactly as the reference
 // loop does (src/masterEquation.c:644,670-679): t += h, then the
 // sub-threshold counter and the exit test.
+template <int NT = kBlock>
 __device__ void finish_step(RunState *st, double *part_chk, double *part_mass,
                             double blk_chk, double blk_mass) {
     __shared__ int s_last;
-    __shared__ double s_red[kWaves];
+    __shared__ double s_red[NT / kWave];
     if (threadIdx.x == 0) {
         part_chk[blockIdx.x] = blk_chk;
         part_mass[blockIdx.x] = blk_mass;
@@ -91,12 +94,12 @@ __device__ void finish_step(RunState *st, double *part_chk, double *part_mass,
     if (!s_last) return;
     __threadfence();
     Kahan kc, km;
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += NT) {
         kc.add(load_relaxed(part_chk + b));
         km.add(load_relaxed(part_mass + b));
     }
-    double chk = block_sum(kc.value(), s_red);
-    double mass = block_sum(km.value(), s_red);
+    double chk = block_sum<NT>(kc.value(), s_red);
+    double mass = block_sum<NT>(km.value(), s_red);
     if (threadIdx.x == 0) {
         st->checksum = chk;
         st->mass = mass;

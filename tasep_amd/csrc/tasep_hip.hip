@@ -16,6 +16,7 @@
 
 #include "../../include/tasep_hip.h"
 #include "kernels.hpp"
+#include "tile_kernels.hpp"
 
 using namespace tasep;
 
@@ -83,6 +84,12 @@ struct tasep_ctx {
     double *pchk = nullptr, *pmass = nullptr;
     double *mom_part = nullptr, *mom_out = nullptr;
     double host_rates[kRateStride] = {0};
+    // options (tasep_set_option)
+    bool tiled = true;          // tiled matrix-free passes where n allows
+    bool xcd_order = true;      // XCD-aware tile order
+    bool force_staged = false;  // staged RK4 even for constant rates
+    int tile_blocks = 0;        // persistent grid of the tiled passes (0: by occupancy)
+    uint64_t steps_before = 0;  // cumulative step count when the current run call began
 };
 
 // --------------------------------------------------------------- utilities
@@ -444,6 +451,99 @@ static void rk4_staged_step(tasep_ctx *c, double h, uint64_t j) {
     launch_pass<V, CSR, true, EPI_RK4>(c, a, "rk4_s4", 5 * e);
 }
 
+// ---- tiled matrix-free passes (n >= b + 1) ----
+template <class V>
+static TileArgs<V> tile_base(tasep_ctx *c) {
+    TileArgs<V> a;
+    memset(&a, 0, sizeof a);
+    a.rates = c->rates;
+    a.st = c->st;
+    a.pchk = c->pchk;
+    a.pmass = c->pmass;
+    a.n = c->n;
+    a.ntiles = c->dim >> TileGeom<V>::b;
+    return a;
+}
+
+// Grid of a tiled pass.  Store passes: 16 workgroups per resident slot
+// (occupancy x CUs), which balances the tail; the final pass of a step: one
+// resident wave of workgroups, so that its end-of-step reduction sees few
+// partials (measured at n=28: 0.92 vs 1.03 ms for the store passes, 1.00 vs
+// 1.14 ms for the final pass).  Rounded down to a multiple of 8 XCDs.
+template <class V, bool TWO, int EPI, bool SELF>
+static int tile_grid_cap() {
+    static int cap = 0;
+    if (!cap) {
+        int per_cu = 0, dev = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_pass<V, TWO, EPI, SELF>,
+                                                         kTileThreads, 0) != hipSuccess ||
+            hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            per_cu < 1 || cus < 1)
+            return 1024;
+        cap = (per_cu * cus * (EPI == EPI_STORE ? 16 : 1)) & ~7;
+        if (cap < 8) cap = 8;
+        if (cap > kMaxBlocks) cap = kMaxBlocks;
+    }
+    return cap;
+}
+
+template <class V, bool TWO, int EPI, bool SELF = false>
+static void launch_tile(tasep_ctx *c, TileArgs<V> a, const char *name, double bytes) {
+    const uint64_t cap = c->tile_blocks > 0 ? (uint64_t)c->tile_blocks
+                                            : (uint64_t)tile_grid_cap<V, TWO, EPI, SELF>();
+    uint64_t g = a.ntiles < cap ? a.ntiles : cap;
+    a.xcd_order = (c->xcd_order && (a.ntiles % 8) == 0 && (g % 8) == 0) ? 1 : 0;
+    prof_begin(c);
+    hipLaunchKernelGGL((k_tile_pass<V, TWO, EPI, SELF>), dim3(g), dim3(kTileThreads), 0, c->stream, a);
+    prof_end(c, name, bytes);
+}
+
+template <class V>
+static void rk4_horner_tile(tasep_ctx *c, double h) {
+    V *x = (V *)xcur(c);
+    V *u = (V *)c->buf[2];
+    V *v = (V *)c->buf[3];
+    TileArgs<V> a = tile_base<V>(c);
+    const double e = (double)sizeof(V);
+    a.x = x;
+    a.use_self = 1;
+    a.gx = x; a.out = u; a.c = (V)(0.25 * h);
+    launch_tile<V, false, EPI_STORE, true>(c, a, "horner_p1", 2 * e);
+    a.gx = u; a.out = v; a.c = (V)(h / 3.0);
+    launch_tile<V, false, EPI_STORE>(c, a, "horner_p23", 3 * e);
+    a.gx = v; a.out = u; a.c = (V)(0.5 * h);
+    launch_tile<V, false, EPI_STORE>(c, a, "horner_p23", 3 * e);
+    a.gx = u; a.out = x; a.c = (V)h;
+    launch_tile<V, false, EPI_HORNER>(c, a, "horner_p4", 3 * e);
+}
+
+template <class V>
+static void rk4_staged_tile(tasep_ctx *c, double h, uint64_t j) {
+    V *x = (V *)c->buf[c->cur ^ (int)(j & 1)];
+    V *xn = (V *)c->buf[c->cur ^ (int)((j + 1) & 1)];
+    V *k1 = (V *)c->buf[2], *k2 = (V *)c->buf[3], *k3 = (V *)c->buf[4];
+    TileArgs<V> a = tile_base<V>(c);
+    const double e = (double)sizeof(V);
+    a.x = x;
+    a.gx = x;
+    a.c = (V)h;
+    a.use_self = 0;
+    a.rset = 0; a.out = k1;
+    launch_tile<V, false, EPI_STORE>(c, a, "rk4_s1", 2 * e);
+    a.rset = 1; a.gk = k1; a.ck = (V)0.5; a.out = k2;
+    launch_tile<V, true, EPI_STORE>(c, a, "rk4_s23", 3 * e);
+    a.rset = 1; a.gk = k2; a.ck = (V)0.5; a.out = k3;
+    launch_tile<V, true, EPI_STORE>(c, a, "rk4_s23", 3 * e);
+    a.rset = 2; a.gk = k3; a.ck = (V)1.0; a.out = xn; a.k1 = k1; a.k2 = k2; a.k3 = k3;
+    launch_tile<V, true, EPI_RK4>(c, a, "rk4_s4", 5 * e);
+}
+
+template <class V>
+static bool use_tile(const tasep_ctx *c) {
+    return c->tiled && c->gen == TASEP_MF && c->n >= TileGeom<V>::b + 1;
+}
+
 static void ref_step(tasep_ctx *c, double h, uint64_t j) {
     RefArgs a;
     a.x = (const double *)c->buf[c->cur ^ (int)(j & 1)];
@@ -468,15 +568,16 @@ static void ref_step(tasep_ctx *c, double h, uint64_t j) {
     }
 }
 
-// Allocate what a run needs and reset the device loop state.
-static int prepare_run(tasep_ctx *c, double t0, double h, double thr, int patience) {
+// Allocate what a run needs and reset the device loop state (resume: keep
+// t, the step count and the early-exit counters of the previous run).
+static int prepare_run(tasep_ctx *c, double t0, double h, double thr, int patience, int resume = 0) {
     if (!c->have_rates) return fail(TASEP_ERR_STATE, "set rates before running");
     if (!isfinite(h) || !isfinite(t0)) return fail(TASEP_ERR_ARG, "t0 and h must be finite");
     if (c->gen == TASEP_CSR && !c->csr_built) {
         int rc = tasep_build_csr(c);
         if (rc) return rc;
     }
-    const bool staged = c->integ == TASEP_RK4 && c->tv;
+    const bool staged = c->integ == TASEP_RK4 && (c->tv || c->force_staged);
     if (c->integ == TASEP_REF || staged) {
         int rc = ensure_buf(c, 1);
         if (rc) return rc;
@@ -491,20 +592,33 @@ static int prepare_run(tasep_ctx *c, double t0, double h, double thr, int patien
     HIP_OK(hipMemcpyAsync(c->st_host, c->st, sizeof(RunState), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     const double scale = c->st_host->scale;
+    c->steps_before = resume ? c->st_host->steps_done : 0;
+    if (resume) {
+        c->st_host->h = h;
+        c->st_host->thr_h = thr * h;
+        c->st_host->patience = c->tv ? -1 : patience;
+        c->st_host->counter = 0;
+        HIP_OK(hipMemcpyAsync(c->st, c->st_host, sizeof(RunState), hipMemcpyHostToDevice, c->stream));
+        return TASEP_OK;
+    }
     memset(c->st_host, 0, sizeof(RunState));
     c->st_host->t = t0;
     c->st_host->h = h;
     c->st_host->thr_h = thr * h;
     c->st_host->patience = c->tv ? -1 : patience;
-    c->st_host->renorm = c->prec == TASEP_F32 ? 1 : 0;
-    c->st_host->scale = c->prec == TASEP_F32 ? scale : 1.0;
+    // The FP32 vector is never rescaled in place (a factor within 1e-9 of 1
+    // is below its resolution); its FP64 mass is tracked every step and
+    // tasep_moments divides by it.
+    c->st_host->renorm = 0;
+    c->st_host->scale = 1.0;
+    (void)scale;
     HIP_OK(hipMemcpyAsync(c->st, c->st_host, sizeof(RunState), hipMemcpyHostToDevice, c->stream));
     return TASEP_OK;
 }
 
 // Enqueue step j of the current run.
 static void enqueue_step(tasep_ctx *c, double h, uint64_t j) {
-    const bool staged = c->integ == TASEP_RK4 && c->tv;
+    const bool staged = c->integ == TASEP_RK4 && (c->tv || c->force_staged);
     if (c->tv) {
         TvArgs tva;
         tva.type = c->tv_type;
@@ -518,10 +632,18 @@ static void enqueue_step(tasep_ctx *c, double h, uint64_t j) {
     if (c->integ == TASEP_REF) {
         ref_step(c, h, j);
     } else if (c->prec == TASEP_F32) {
-        if (staged) rk4_staged_step<float, false>(c, h, j);
-        else rk4_horner_step<float, false>(c, h);
+        if (use_tile<float>(c)) {
+            if (staged) rk4_staged_tile<float>(c, h, j);
+            else rk4_horner_tile<float>(c, h);
+        } else {
+            if (staged) rk4_staged_step<float, false>(c, h, j);
+            else rk4_horner_step<float, false>(c, h);
+        }
     } else if (c->gen == TASEP_CSR) {
         rk4_horner_step<double, true>(c, h);
+    } else if (use_tile<double>(c)) {
+        if (staged) rk4_staged_tile<double>(c, h, j);
+        else rk4_horner_tile<double>(c, h);
     } else {
         if (staged) rk4_staged_step<double, false>(c, h, j);
         else rk4_horner_step<double, false>(c, h);
@@ -533,8 +655,9 @@ static int finish_run(tasep_ctx *c, tasep_run_info *info) {
     HIP_OK(hipMemcpyAsync(c->st_host, c->st, sizeof(RunState), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     const RunState &s = *c->st_host;
-    const bool staged = c->integ == TASEP_RK4 && c->tv;
-    if (c->integ == TASEP_REF || staged) c->cur ^= (int)(s.steps_done & 1);
+    const bool staged = c->integ == TASEP_RK4 && (c->tv || c->force_staged);
+    if (c->integ == TASEP_REF || staged) c->cur ^= (int)((s.steps_done - c->steps_before) & 1);
+    c->steps_before = s.steps_done;
     if (info) {
         info->steps_done = s.steps_done;
         info->steps_at_stop = s.steps_at_stop;
@@ -551,8 +674,13 @@ extern "C" {
 
 int tasep_run(tasep_ctx *c, double t0, double h, uint64_t steps, double thr, int patience,
               tasep_run_info *info) {
+    return tasep_run_resume(c, t0, h, steps, thr, patience, 0, info);
+}
+
+int tasep_run_resume(tasep_ctx *c, double t0, double h, uint64_t steps, double thr, int patience,
+                     int resume, tasep_run_info *info) {
     if (!c) return fail(TASEP_ERR_ARG, "NULL context");
-    int rc = prepare_run(c, t0, h, thr, patience);
+    int rc = prepare_run(c, t0, h, thr, patience, resume);
     if (rc) return rc;
     const bool watch = c->st_host->patience >= 0;
     // steps per host check of the exit flag: about 2^26 state-updates
@@ -572,6 +700,22 @@ int tasep_run(tasep_ctx *c, double t0, double h, uint64_t steps, double thr, int
         }
     }
     return finish_run(c, info);
+}
+
+int tasep_set_option(tasep_ctx *c, int key, int64_t value) {
+    if (!c) return fail(TASEP_ERR_ARG, "NULL context");
+    switch (key) {
+    case TASEP_OPT_TILED: c->tiled = value != 0; break;
+    case TASEP_OPT_XCD_ORDER: c->xcd_order = value != 0; break;
+    case TASEP_OPT_RK4_STAGED: c->force_staged = value != 0; break;
+    case TASEP_OPT_TILE_BLOCKS:
+        if (value != 0 && (value < 8 || value > kMaxBlocks || value % 8))
+            return fail(TASEP_ERR_ARG, "tile blocks: 0 or a multiple of 8 in [8, 8192]");
+        c->tile_blocks = (int)value;
+        break;
+    default: return fail(TASEP_ERR_ARG, "unknown option");
+    }
+    return TASEP_OK;
 }
 
 int tasep_step_profile(tasep_ctx *c, double t0, double h, int reps, tasep_kernel_time *out, int max,

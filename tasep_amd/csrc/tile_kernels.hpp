@@ -1,0 +1,439 @@
+// tile_kernels.hpp -- matrix-free generator passes on 2-D state tiles.
+//
+// A workgroup of 512 lanes (8 waves) owns a tile of 8 chunks; chunk w (wave
+// w) holds C = 4 columns of 64*E consecutive states (E = 16/sizeof(V) per
+// lane per column, one 16-byte load), the columns 2^LB states apart, and the
+// 8 chunks differ only in the top three state bits.  With e = log2 E,
+// LB = e + 6 and TB0 = LB + 2 the bits of a state are
+//   bits 0 .. e-1        register index j
+//   bits e .. LB-1       lane l (6 bits)
+//   bits LB, LB+1        column c (registers)
+//   bits TB0 .. n-4      tile index T (uniform over the workgroup)
+//   bits n-3 .. n-1      wave w (the chunk)
+// so that the entry site and the two bonds at the top of the lattice --
+// whose neighbours are 2^(n-1) and 2^(n-2) states away, too far for any
+// cache -- are exchanged through LDS.  Bond k (bits k, k+1):
+//   k <= e-2          both register bits: static, no loads
+//   k == e-1          register bit + lane bit 0: next lane via LDS
+//   e <= k <= LB-2    lane bits: per-lane weight, lane l + 2^(k-e) via LDS
+//   k == LB-1         lane bit 5 + column bit 0: lane l-32 of column c+1 (LDS)
+//   k == LB           column bits: static, register column c+1
+//   k == LB+1         column bit 1 + tile bit 0: columns 2, 3 read columns
+//                     0, 1 of tile T+1 (global, when the tile bit is 0)
+//   TB0 <= k <= n-5   tile bits: uniform, the whole tile reads tile
+//                     T + 2^(k-TB0) (coalesced 16-byte loads)
+//   k == n-4          tile top bit + wave bit 0: per-wave, chunk w+1 of tile
+//                     T - 2^(n-4-TB0), 2^(n-4) states away
+//   k == n-3, n-2     wave bits: chunk w+1 / w+2 of the same tile via LDS
+//   entry (n-1)       wave bit 2: chunk w-4 via LDS
+// The diagonal splits the same way (static, per-lane, per-wave, per-tile).
+// Every global neighbour access is a contiguous stream at most 2^(n-4)
+// states away; only active tile bonds are read (two streams in flight), and
+// the per-tile uniform work is shared by 16 states per lane.
+//
+// Memory-level parallelism: the workgroup is persistent over its tiles; the
+// next tile's own vector is prefetched while the current one is computed,
+// the self vector and the first neighbour stream are issued before the LDS
+// phase, and the remaining neighbour streams are double-buffered.
+//
+// Tiles are walked in an XCD-aware order: workgroup g runs on XCD g % 8 and
+// takes its tiles from a contiguous eighth of the tile range.
+#pragma once
+
+#include "device_common.hpp"
+#include "generator.hpp"
+
+namespace tasep {
+
+constexpr int kTileThreads = 512;
+constexpr int kTileWaves = kTileThreads / kWave;
+
+template <class V> struct TileGeom;
+// E states per lane per column (one 16-byte load), C = 4 columns per lane.
+// Tile bits start at TB0 = LB + 2; a tile holds 2^b = 2^(TB0 + 3) states.
+template <> struct TileGeom<float> {
+    static constexpr int E = 4, e = 2, LB = 8, C = 4, TB0 = LB + 2, b = TB0 + 3;  // 8192 states
+};
+template <> struct TileGeom<double> {
+    static constexpr int E = 2, e = 1, LB = 7, C = 4, TB0 = LB + 2, b = TB0 + 3;  // 4096 states
+};
+
+__device__ __forceinline__ void ld16(const float *__restrict__ p, float (&o)[4]) {
+    const float4 a = *reinterpret_cast<const float4 *>(p);
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+}
+__device__ __forceinline__ void ld16(const double *__restrict__ p, double (&o)[2]) {
+    const double2 a = *reinterpret_cast<const double2 *>(p);
+    o[0] = a.x; o[1] = a.y;
+}
+__device__ __forceinline__ void st16(float *__restrict__ p, const float (&o)[4]) {
+    *reinterpret_cast<float4 *>(p) = make_float4(o[0], o[1], o[2], o[3]);
+}
+__device__ __forceinline__ void st16(double *__restrict__ p, const double (&o)[2]) {
+    *reinterpret_cast<double2 *>(p) = make_double2(o[0], o[1]);
+}
+
+// Streaming (non-temporal) variants for vectors touched once per pass: the
+// self-term read of x and the output write.  Keeping them out of the way in
+// L2 leaves its capacity to the gathered vector, which neighbouring tiles
+// re-read.
+#ifndef TASEP_NT_OUT
+#define TASEP_NT_OUT 1
+#endif
+#ifndef TASEP_NT_SELF
+#define TASEP_NT_SELF 1
+#endif
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void ld16_nt(const float *p, float (&o)[4]) {
+    const f32x4 a = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p));
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+}
+__device__ __forceinline__ void ld16_nt(const double *p, double (&o)[2]) {
+    const f64x2 a = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(p));
+    o[0] = a.x; o[1] = a.y;
+}
+__device__ __forceinline__ void st16_nt(float *p, const float (&o)[4]) {
+    f32x4 v = {o[0], o[1], o[2], o[3]};
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(p));
+}
+__device__ __forceinline__ void st16_nt(double *p, const double (&o)[2]) {
+    f64x2 v = {o[0], o[1]};
+    __builtin_nontemporal_store(v, reinterpret_cast<f64x2 *>(p));
+}
+template <class V, int E>
+__device__ __forceinline__ void ld_self(const V *p, V (&o)[E]) {
+    if constexpr (TASEP_NT_SELF) ld16_nt(p, o);
+    else ld16(p, o);
+}
+template <class V, int E>
+__device__ __forceinline__ void st_out(V *p, const V (&o)[E]) {
+    if constexpr (TASEP_NT_OUT) st16_nt(p, o);
+    else st16(p, o);
+}
+
+template <class V>
+struct TileArgs {
+    const V *gx;       // gathered vector
+    const V *gk;       // second gathered vector (staged RK4)
+    V ck;              // G = gx + ck * gk
+    const V *x;        // state, self term
+    V *out;
+    const V *k1, *k2, *k3;
+    V c;               // out = self + c * (L G)
+    int use_self;      // STORE: add X[s] (or G[s] when SELF)
+    const double *rates;
+    int rset;
+    RunState *st;
+    double *pchk, *pmass;
+    int n;
+    uint64_t ntiles;
+    int xcd_order;     // 1: XCD-aware tile order
+};
+
+
+// Load E*C stage-input values of one lane: C columns 2^LB states apart,
+// starting at the wave-uniform base + lane offset lo (G = gx [+ ck*gk]).
+template <class V, bool TWO, int E, int C, int LB>
+__device__ __forceinline__ void load_cols(const TileArgs<V> &a, uint64_t base, uint32_t lo,
+                                          V (&g)[C][E]) {
+#pragma unroll
+    for (int c = 0; c < C; c++) ld16(a.gx + base + lo + ((uint32_t)c << LB), g[c]);
+    if constexpr (TWO) {
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            V q[E];
+            ld16(a.gk + base + lo + ((uint32_t)c << LB), q);
+#pragma unroll
+            for (int j = 0; j < E; j++) g[c][j] = fma(a.ck, q[j], g[c][j]);
+        }
+    }
+}
+
+template <class V, int E, int C, int LB>
+__device__ __forceinline__ void load_cols_x(const V *p, uint64_t base, uint32_t lo, V (&g)[C][E]) {
+#pragma unroll
+    for (int c = 0; c < C; c++) ld_self<V, E>(p + base + lo + ((uint32_t)c << LB), g[c]);
+}
+
+template <class V, int E, int C>
+__device__ __forceinline__ void fma_cols(V r, const V (&nb)[C][E], V (&acc)[C][E]) {
+#pragma unroll
+    for (int c = 0; c < C; c++)
+#pragma unroll
+        for (int j = 0; j < E; j++) acc[c][j] = fma(r, nb[c][j], acc[c][j]);
+}
+
+__device__ __forceinline__ uint64_t tile_of(uint64_t it, uint64_t per_xcd, int xcd_order) {
+    // it = g + 8*slot for workgroup g's tiles when gridDim % 8 == 0
+    return xcd_order ? (it & 7u) * per_xcd + (it >> 3) : it;
+}
+
+// Pop the lowest set bit of a uniform bond mask: returns the bond index
+// (LB0 + bit) or -1 when the mask is empty.
+__device__ __forceinline__ int pop_bond(uint64_t &m, int LB0) {
+    if (!m) return -1;
+    const int k = LB0 + __builtin_ctzll(m);
+    m &= m - 1;
+    return k;
+}
+
+template <class V, bool TWO, int EPI, bool SELF>
+__global__ __launch_bounds__(kTileThreads) void k_tile_pass(const TileArgs<V> a) {
+    using G = TileGeom<V>;
+    constexpr int E = G::E, e = G::e, LB = G::LB, C = G::C, TB0 = G::TB0;
+    if (a.st->stopped) return;
+    __shared__ V tile[kTileThreads * C * E];
+    __shared__ V rr[kRateStride];
+    __shared__ double red[kTileWaves];
+    const int n = a.n;
+    if (threadIdx.x <= n) rr[threadIdx.x] = (V)a.rates[a.rset * kRateStride + threadIdx.x];
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int l = t & 63;                                          // lane in the chunk
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);          // chunk (uniform per wave)
+    const uint32_t lo = (uint32_t)l * E;                           // lane offset (elements)
+    const V alpha = rr[0], beta = rr[n];
+    const int tb = n - 3 - TB0;                                    // bits of the tile index (>= 1)
+    const uint64_t tmask = tb > 1 ? (((uint64_t)1 << (tb - 1)) - 1) : 0;  // bonds TB0..n-5
+    const uint64_t wofs = (uint64_t)w << (n - 3);
+    // LDS slot of (wave, column, lane): ((w*C + c)*64 + l)*E
+    const uint32_t lds0 = ((uint32_t)w * C * 64 + (uint32_t)l) * E;
+    constexpr uint32_t LCOL = 64 * E;                              // next column
+    // ---- tile-invariant parts of the diagonal: lane bonds, wave bonds, entry
+    const uint32_t lout = ~(uint32_t)l & ((uint32_t)l >> 1) & 0x1Fu;
+    V d_fix = 0;
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+        if ((lout >> q) & 1) d_fix += rr[n - 1 - (e + q)];
+    const bool w0 = w & 1, w1 = (w >> 1) & 1, w2 = (w >> 2) & 1;
+    const V r_n2 = rr[1], r_n3 = rr[2], r_n4 = rr[3];           // bonds n-2, n-3, n-4
+    if (!w0 && w1) d_fix += r_n3;
+    if (!w1 && w2) d_fix += r_n2;
+    if (!w2) d_fix += alpha;
+    V wl[5];                                                       // lane-bond in-flow weights
+    uint32_t ls[5];                                                // and their LDS sources
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const bool on = ((l >> q) & 1) && !((l >> (q + 1)) & 1);
+        wl[q] = on ? rr[n - 1 - (e + q)] : (V)0;
+        ls[q] = on ? lds0 + ((uint32_t)E << q) : lds0;
+    }
+    const V r_e1 = rr[n - e];                                      // bond e-1
+    const bool lane_hi = (l >> 5) & 1;
+    const V r_c0 = rr[n - LB];                                     // bond LB-1: lane bit 5, column bit 0
+    const V r_c1 = rr[n - 1 - LB];                                 // bond LB: column bits 0, 1
+    const V r_c2 = rr[n - 2 - LB];                                 // bond LB+1: column bit 1, tile bit 0
+    const V w_c0 = lane_hi ? r_c0 : (V)0;                          // in-flow for columns 0, 2
+    const V d_c0 = lane_hi ? (V)0 : r_c0;                          // out-flow for columns 1, 3
+    const bool use_self = (EPI != EPI_STORE) || (!SELF && a.use_self);
+    Kahan chk, mass;
+    const uint64_t per_xcd = a.ntiles >> 3;
+    const uint64_t stride = gridDim.x;
+    const uint64_t last = a.ntiles - 1;
+
+    uint64_t it = blockIdx.x;
+    V g[C][E];
+    if (it < a.ntiles)  // prologue: own vector of the first tile
+        load_cols<V, TWO, E, C, LB>(a, (tile_of(it, per_xcd, a.xcd_order) << TB0) + wofs, lo, g);
+    for (; it < a.ntiles; it += stride) {
+        const uint64_t T = tile_of(it, per_xcd, a.xcd_order);
+        const uint64_t wb = (T << TB0) + wofs;                     // wave base (uniform)
+        const bool t_lo = T & 1, t_hi = (T >> (tb - 1)) & 1;
+        // ---- issue: first tile-bond stream, self vector, next tile's own vector
+        uint64_t mm = T & ~(T >> 1) & tmask;
+        int kA = pop_bond(mm, TB0);
+        V A[C][E], B[C][E];
+        load_cols<V, TWO, E, C, LB>(a, wb + (kA >= 0 ? ((uint64_t)1 << kA) : 0), lo, A);
+        V xs[C][E];
+        if constexpr (EPI != EPI_STORE || !SELF) load_cols_x<V, E, C, LB>(a.x, wb, lo, xs);
+        V gn[C][E];
+        {
+            const uint64_t itn = it + stride <= last ? it + stride : it;
+            load_cols<V, TWO, E, C, LB>(a, (tile_of(itn, per_xcd, a.xcd_order) << TB0) + wofs, lo, gn);
+        }
+        // ---- own vector to LDS
+        __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+        for (int c = 0; c < C; c++) st16(&tile[lds0 + c * LCOL], g[c]);
+        // ---- uniform and per-wave parts of the diagonal
+        const uint64_t hout = ~T & (T >> 1) & tmask;
+        V d_t = d_fix;
+        for (uint64_t m = hout; m; m &= m - 1) d_t += rr[n - 1 - (TB0 + __builtin_ctzll(m))];
+        if (!t_hi && w0) d_t += r_n4;
+        __syncthreads();  // tile in LDS
+        V acc[C][E];
+#pragma unroll
+        for (int c = 0; c < C; c++)
+#pragma unroll
+            for (int j = 0; j < E; j++) acc[c][j] = 0;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const uint32_t base = lds0 + c * LCOL;
+            V lv[E];
+            // lane bonds e..LB-2 (weight 0 where inactive)
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                ld16(&tile[ls[q] + c * LCOL], lv);
+#pragma unroll
+                for (int j = 0; j < E; j++) acc[c][j] = fma(wl[q], lv[j], acc[c][j]);
+            }
+            // bond e-1: register bit e-1, lane bit 0 (next lane)
+            ld16(&tile[(l & 1) ? base : base + E], lv);
+            const V we = (l & 1) ? (V)0 : r_e1;
+#pragma unroll
+            for (int j = E / 2; j < E; j++) acc[c][j] = fma(we, lv[j - E / 2], acc[c][j]);
+            // bond LB-1: lane bit 5, column bit 0 (columns 0, 2 read lane l-32 of column c+1)
+            if ((c & 1) == 0) {
+                ld16(&tile[lane_hi ? base + LCOL - 32 * E : base], lv);
+#pragma unroll
+                for (int j = 0; j < E; j++) acc[c][j] = fma(w_c0, lv[j], acc[c][j]);
+            }
+        }
+        // ---- wave bonds n-3, n-2 and the entry: other chunks, same lane/column
+        if (w0 && !w1) {
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                V lv[E];
+                ld16(&tile[lds0 + c * LCOL + C * LCOL], lv);
+#pragma unroll
+                for (int j = 0; j < E; j++) acc[c][j] = fma(r_n3, lv[j], acc[c][j]);
+            }
+        }
+        if (w1 && !w2) {
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                V lv[E];
+                ld16(&tile[lds0 + c * LCOL + 2 * C * LCOL], lv);
+#pragma unroll
+                for (int j = 0; j < E; j++) acc[c][j] = fma(r_n2, lv[j], acc[c][j]);
+            }
+        }
+        if (w2) {
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                V lv[E];
+                ld16(&tile[lds0 + c * LCOL - 4 * C * LCOL], lv);
+#pragma unroll
+                for (int j = 0; j < E; j++) acc[c][j] = fma(alpha, lv[j], acc[c][j]);
+            }
+        }
+        // ---- static register bonds: 0..e-2, column bond LB, exit site
+#pragma unroll
+        for (int k = 0; k <= e - 2; k++) {
+            const V r = rr[n - 1 - k];
+#pragma unroll
+            for (int c = 0; c < C; c++)
+#pragma unroll
+                for (int j = 0; j < E; j++)
+                    if (((j >> k) & 1) && !((j >> (k + 1)) & 1)) acc[c][j] = fma(r, g[c][j + (1 << k)], acc[c][j]);
+        }
+#pragma unroll
+        for (int j = 0; j < E; j++) acc[1][j] = fma(r_c1, g[2][j], acc[1][j]);
+#pragma unroll
+        for (int c = 0; c < C; c++)
+#pragma unroll
+            for (int j = 0; j < E; j += 2) acc[c][j] = fma(beta, g[c][j + 1], acc[c][j]);
+        // ---- global streams: bond LB+1 (columns 2, 3 from tile T+1), bond
+        // n-4 (per wave), then the tile bonds, two in flight at a time
+        if (!t_lo) {
+            V nb[2][E];
+            ld16(a.gx + wb + lo + ((uint64_t)1 << TB0), nb[0]);
+            ld16(a.gx + wb + lo + ((uint64_t)1 << TB0) + (1u << LB), nb[1]);
+            if constexpr (TWO) {
+                V q0[E], q1[E];
+                ld16(a.gk + wb + lo + ((uint64_t)1 << TB0), q0);
+                ld16(a.gk + wb + lo + ((uint64_t)1 << TB0) + (1u << LB), q1);
+#pragma unroll
+                for (int j = 0; j < E; j++) {
+                    nb[0][j] = fma(a.ck, q0[j], nb[0][j]);
+                    nb[1][j] = fma(a.ck, q1[j], nb[1][j]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < E; j++) {
+                acc[2][j] = fma(r_c2, nb[0][j], acc[2][j]);
+                acc[3][j] = fma(r_c2, nb[1][j], acc[3][j]);
+            }
+        }
+        if (t_hi && !w0) {
+            V nb[C][E];
+            load_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << (n - 4)), lo, nb);
+            fma_cols<V, E, C>(r_n4, nb, acc);
+        }
+        while (kA >= 0) {
+            const int kB = pop_bond(mm, TB0);
+            load_cols<V, TWO, E, C, LB>(a, wb + (kB >= 0 ? ((uint64_t)1 << kB) : 0), lo, B);
+            fma_cols<V, E, C>(rr[n - 1 - kA], A, acc);
+            if (kB < 0) break;
+            kA = pop_bond(mm, TB0);
+            load_cols<V, TWO, E, C, LB>(a, wb + (kA >= 0 ? ((uint64_t)1 << kA) : 0), lo, A);
+            fma_cols<V, E, C>(rr[n - 1 - kB], B, acc);
+        }
+        // ---- diagonal and epilogue
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            V dc = d_t;
+            if (c & 1) dc += d_c0;                                 // bond LB-1 out-flow
+            if (c == 2) dc += r_c1;                                // bond LB out-flow
+            if (c < 2 && t_lo) dc += r_c2;                         // bond LB+1 out-flow
+            V d[E];
+#pragma unroll
+            for (int j = 0; j < E; j++) {
+                d[j] = dc;
+                if (j & 1) d[j] += beta;
+                if (j < E / 2 && (l & 1)) d[j] += r_e1;
+            }
+#pragma unroll
+            for (int k = 0; k <= e - 2; k++) {
+                const V r = rr[n - 1 - k];
+#pragma unroll
+                for (int j = 0; j < E; j++)
+                    if (!((j >> k) & 1) && ((j >> (k + 1)) & 1)) d[j] += r;
+            }
+            V o[E];
+            if constexpr (EPI == EPI_STORE) {
+#pragma unroll
+                for (int j = 0; j < E; j++) {
+                    const V self = SELF ? g[c][j] : (use_self ? xs[c][j] : (V)0);
+                    o[j] = fma(a.c, fma(-d[j], g[c][j], acc[c][j]), self);
+                }
+            } else {
+                V dl[E];
+#pragma unroll
+                for (int j = 0; j < E; j++) dl[j] = a.c * fma(-d[j], g[c][j], acc[c][j]);
+                if constexpr (EPI == EPI_RK4) {
+                    V q1[E], q2[E], q3[E];
+                    const uint64_t s = wb + lo + ((uint32_t)c << LB);
+                    ld16(a.k1 + s, q1);
+                    ld16(a.k2 + s, q2);
+                    ld16(a.k3 + s, q3);
+#pragma unroll
+                    for (int j = 0; j < E; j++)
+                        dl[j] = (((q1[j] + (V)2 * q2[j]) + (V)2 * q3[j]) + dl[j]) / (V)6;
+                }
+                double cs = 0, ms = 0;
+#pragma unroll
+                for (int j = 0; j < E; j++) {
+                    o[j] = xs[c][j] + dl[j];
+                    cs += fabs((double)dl[j]);
+                    ms += (double)o[j];
+                }
+                chk.add(cs);
+                mass.add(ms);
+            }
+            st_out<V, E>(a.out + wb + lo + ((uint32_t)c << LB), o);
+        }
+#pragma unroll
+        for (int c = 0; c < C; c++)
+#pragma unroll
+            for (int j = 0; j < E; j++) g[c][j] = gn[c][j];
+    }
+    if constexpr (EPI != EPI_STORE) {
+        const double bc = block_sum<kTileThreads>(chk.value(), red);
+        const double bm = block_sum<kTileThreads>(mass.value(), red);
+        finish_step<kTileThreads>(a.st, a.pchk, a.pmass, bc, bm);
+    }
+}
+
+}  // namespace tasep

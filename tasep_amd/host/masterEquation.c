@@ -1,0 +1,379 @@
+/*
+ * masterEquation -- command-line host of the gfx950 engine.
+ *
+ * Same command line, prompts, stdout summary lines and output files as the
+ * reference driver (KilianPiochUBT/TASEP src/masterEquation.c:145-1578);
+ * the compute section (generator build, time stepping, early exit, moments)
+ * runs on the GPU through the C ABI of include/tasep_hip.h.  The integrator
+ * is the reference's loop (TASEP_REF, FP64) so results match the reference
+ * binary.  Environment knobs that the reference does not have (no new CLI
+ * options): TASEP_INTEGRATOR=rk4 and TASEP_PRECISION=f32 select classical
+ * RK4 and the FP32 state vector.
+ *
+ * Deliberate differences from the reference, all where it has undefined
+ * behaviour or crashes:
+ *   - --gamma frees nothing (the reference frees the alpha string twice,
+ *     src/masterEquation.c:330);
+ *   - --time-var without -i/--alpha runs with the default rate functions
+ *     (the reference dereferences an unallocated array, :383,886);
+ *   - --save-select without 'p' names files as if 'p' were selected (the
+ *     reference reads an uninitialised length, :1017,1122,1285);
+ *   - _highcorr holds the normalised correlation the reference computes
+ *     but never returns (:47-75, printed with a mismatched format, :1445).
+ */
+#include <getopt.h>
+#include <inttypes.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <sys/time.h>
+#include <time.h>
+
+#include "../../include/tasep_hip.h"
+#include "writers.h"
+
+static const struct option kLongOpts[] = {
+    {"time-var", no_argument, NULL, 0},
+    {"function-type", required_argument, NULL, 1},
+    {"offset", required_argument, NULL, 2},
+    {"amplitude", required_argument, NULL, 3},
+    {"period", required_argument, NULL, 4},
+    {"constant", required_argument, NULL, 5},
+    {"save-select", required_argument, NULL, 6},
+    {"thread-num", required_argument, NULL, 7},
+    {"alpha", required_argument, NULL, 8},
+    {"beta", required_argument, NULL, 9},
+    {"gamma", required_argument, NULL, 10},
+    {NULL, 0, NULL, 0}};
+
+static void usage(const char *prog)
+{
+    printf("Usage: %s\n-i jump rate file\n-n chain length\n-o output file name\n-t start time\n-T endtime\n"
+           "-s steps\n-f fast option (caution: this requires more memory)\n"
+           "-a space between snapshots\n"
+           "--time-var sets jump rates to time variant. CAUTION: This cannot be used in combination with -f\n"
+           "--function-type <char* filename with sin,cos,lin in order> sets function type of jump rate (when using <lin> --period option is not used)\n"
+           "--amplitude <char* filename> reads amplitudes from given file and sets functions, when using <lin> these serve as the gradient\n"
+           "--period <char* filename> reads period multipliers from given file an sets jump rate functions. (default is 1)\n"
+           "--offset <char* filename> reads offsets from given file and sets jump rate functions. (default is 0)\n"
+           "--constant <char* filename> reads constants from given file and sets jump rate functions. (default is 0)\n"
+           "--thread-num <char* number_of_threads_to_use> sets number of threads used during parallel portion of the code. Only applies if -f is also selected. (default is max Threads)\n"
+           "--save-select <char* shorts for what to save> shorts are: 'a': all 'e': expected values 'p': probabilities 'c': covariance Matrices 'm': multiplicative expected values\n"
+           "--alpha <float> entry flow rate\n"
+           "--beta <float> exit flow rate\n"
+           "--gamma <float> flow rate inside lattice\n"
+           "-h\n",
+           prog);
+}
+
+/* timestamped error line and exit, like err_exit (src/lib/io.c:13-19) */
+static void die(const char *msg, int code)
+{
+    char stamp[128];
+    time_t now = time(NULL);
+    strftime(stamp, sizeof stamp, "%c", localtime(&now));
+    fprintf(stderr, "[%s]:%s\n", stamp, msg);
+    exit(code);
+}
+
+static void gpu_check(int rc, const char *what)
+{
+    if (rc != TASEP_OK) {
+        fprintf(stderr, "ERROR: %s: %s\n", what, tasep_strerror(rc));
+        exit(-1);
+    }
+}
+
+/* n+1 whitespace-separated doubles; `sep` is the scanf format used by the
+ * reference for that file (src/masterEquation.c:442 "%lf", :762 "%lf, "). */
+static double *read_vector(const char *path, int len, const char *fmt, const char *eof_msg)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) return NULL;
+    double *v = (double *)malloc((size_t)len * sizeof(double));
+    if (!v) die("ERROR: Not enough memory to allocate probability array. ABORTING", -1);
+    for (int i = 0; i < len; i++)
+        if (fscanf(f, fmt, &v[i]) == EOF) die(eof_msg, -1);
+    fclose(f);
+    return v;
+}
+
+static double now_s(void)
+{
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    return (double)tv.tv_sec + (double)tv.tv_usec * 1e-6;
+}
+
+int main(int argc, char *argv[])
+{
+    char *i_file = NULL, *o_file = NULL, *save_sel = NULL;
+    char *tv_file[5] = {NULL, NULL, NULL, NULL, NULL}; /* type, offset, amp, period, const */
+    int have_n = 0, have_t = 0, have_T = 0, have_s = 0, fast = 0, have_a = 0, tvar = 0;
+    int n = 0, anim_res = 0, threads = 1;
+    unsigned int steps = 0;
+    double t = 0, T = 0, alpha = 0, beta = 0, gamma = 0;
+    int have_alpha = 0, have_beta = 0, have_gamma = 0;
+    int opt, idx = 0;
+
+    while ((opt = getopt_long(argc, argv, "i:v:o:t:T:n:s:a:fh?", kLongOpts, &idx)) != -1) {
+        switch (opt) {
+        case 'i': i_file = strdup(optarg); break;
+        case 'o': o_file = strdup(optarg); break;
+        case 't': have_t = 1; t = atof(optarg); break;
+        case 'T': have_T = 1; T = atof(optarg); break;
+        case 'n': have_n = 1; n = atoi(optarg); break;
+        case 's': have_s = 1; steps = (unsigned int)atoi(optarg); break;
+        case 'f': fast = 1; break;
+        case 'a': have_a = 1; anim_res = atoi(optarg); break;
+        case 0: tvar = 1; break;
+        case 1: case 2: case 3: case 4: case 5: tv_file[opt - 1] = strdup(optarg); break;
+        case 6: save_sel = strdup(optarg); break;
+        case 7: threads = atoi(optarg); break;
+        case 8: alpha = atof(optarg); have_alpha = 1; break;
+        case 9: beta = atof(optarg); have_beta = 1; break;
+        case 10: gamma = atof(optarg); have_gamma = 1; break;
+        default: usage(argv[0]); exit(-1);
+        }
+    }
+    if (fast && tvar) die("ERROR: -f cannot be selected in combination with --time-var", -1);
+    if (!have_n) {
+        printf("Enter length: ");
+        if (fscanf(stdin, "%d", &n) == EOF) die("ERROR: Issue while reading n from stdin", -1);
+    }
+    if (n < 2 || n > 34) die("ERROR: lattice size must be in [2, 34]", -1);
+    const uint64_t dim = (uint64_t)1 << n;
+    printf("Allocated result-vector\n");
+
+    double *rates = NULL;
+    if (have_alpha || have_beta || have_gamma) {
+        if (!have_alpha) {
+            fprintf(stderr, "ERROR: No value for alpha found. Please ammend option --alpha to command line options. Exiting\n");
+            exit(-1);
+        }
+        fprintf(stdout, "Found value for\n   alpha: %lf\n", alpha);
+        if (!have_gamma) {
+            fprintf(stderr, "CAUTION: No value for gamma found. If you want to specify gamma, please ammend option --gamma to command line options.\n");
+            gamma = 1;
+            have_gamma = 1;
+        }
+        fprintf(stdout, "   gamma: %lf\n", gamma);
+        if (!have_beta) {
+            fprintf(stderr, "ERROR: No value for beta found. Please ammend option --beta to command line options. Exiting\n");
+            exit(-1);
+        }
+        fprintf(stdout, "   beta: %lf\n", beta);
+    } else if (!i_file && !tvar) {
+        printf("Enter %d probabilities: ", n + 1);
+        rates = (double *)malloc(((size_t)n + 1) * sizeof(double));
+        for (int k = 0; k <= n; k++)
+            if (scanf("%lf", &rates[k]) == EOF) die("ERROR: Could not read from stdin.", -1);
+    }
+    if (!have_T) {
+        printf("No end time found. Please enter: ");
+        if (scanf("%lf", &T) == EOF) die("ERROR: Issue while reading end time from stdin", -1);
+    }
+    if (!have_t) {
+        printf("No start time found. Please enter: ");
+        if (scanf("%lf", &t) == EOF) die("ERROR: Issue while reading start time from stdin", -1);
+    }
+    printf("Generated inital vector with identical values(%.30g)\n", 1. / (double)dim);
+    if (!o_file) {
+        o_file = strdup("generic_output.vec");
+        printf("WARNING: No output file found. Generating default output file ('generic_output.vec').\n");
+        FILE *f = fopen(o_file, "w");
+        if (!f) {
+            printf("ERROR: Cannot open generic output file. Check permissions.\n");
+            exit(-1);
+        }
+        fclose(f);
+    }
+    if (!have_s) {
+        printf("No step number found. Please Enter: ");
+        if (scanf("%u", &steps) == EOF) die("ERROR: Issue while reading step number from stdin", -1);
+    }
+    if (i_file) {
+        double *r = read_vector(i_file, n + 1, "%lf", "ERROR: Issue while reading from jump rate file name");
+        if (r) {
+            free(rates);
+            rates = r;
+        }
+        if (have_alpha || have_beta || have_gamma)
+            fprintf(stdout, "CAUTION: Switch '-i' overrides values given for alpha, beta and gamma.\n");
+    } else if (have_alpha && have_beta) {
+        rates = (double *)malloc(((size_t)n + 1) * sizeof(double));
+        rates[0] = alpha;
+        rates[n] = beta;
+        for (int k = 1; k < n; k++) rates[k] = gamma;
+    }
+    printf("Allocated calculation vectors\n");
+    printf("Number of threads to be used: %d\n", threads);
+
+    /* save selection (src/masterEquation.c:551-583) */
+    int save_e = 0, save_p = 0, save_c = 0, save_m = 0;
+    if (save_sel) {
+        for (size_t k = 0; k < strlen(save_sel); k++) {
+            switch (save_sel[k]) {
+            case 'a': save_e = save_p = save_c = 1; break;
+            case 'e': save_e = 1; break;
+            case 'p': save_p = 1; break;
+            case 'c': save_c = 1; break;
+            case 'm': save_m = 1; break;
+            default: printf("Could not parse save selction\n");
+            }
+        }
+    } else {
+        save_e = save_p = save_c = save_m = 1;
+    }
+
+    /* ---- engine */
+    const double h = (double)((T - t) / ((long double)steps));
+    const char *integ_env = getenv("TASEP_INTEGRATOR");
+    const char *prec_env = getenv("TASEP_PRECISION");
+    const int integ = (integ_env && !strcmp(integ_env, "rk4")) ? TASEP_RK4 : TASEP_REF;
+    const int prec = (integ == TASEP_RK4 && prec_env && !strcmp(prec_env, "f32")) ? TASEP_F32 : TASEP_F64;
+    tasep_ctx *ctx = NULL;
+    gpu_check(tasep_ctx_create(&ctx, n, prec, integ, fast ? TASEP_CSR : TASEP_MF), "creating the GPU context");
+
+    int32_t *tv_type = NULL;
+    double *tv_par[4] = {NULL, NULL, NULL, NULL}; /* offset, amp, period, const */
+    if (tvar) {
+        tv_type = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+        if (tv_file[0]) {
+            FILE *f = fopen(tv_file[0], "r");
+            if (!f) {
+                printf("ERROR: Cannot open file containing function Types. Aborting\n");
+                exit(-1);
+            }
+            char word[256];
+            for (int k = 0; k <= n; k++) {
+                if (fscanf(f, "%255s", word) == EOF)
+                    die("ERROR: Issue while reading function type file: Reached early EOF", -1);
+                switch (word[0]) {
+                case 's': tv_type[k] = 0; break;
+                case 'c': tv_type[k] = 1; break;
+                case 'l': tv_type[k] = 2; break;
+                default:
+                    printf("ERROR: Non identifiable function type fonud in%s. Read %s. Aborting!\n", tv_file[0], word);
+                }
+            }
+            fclose(f);
+            printf("Read function types successfully\n");
+        } else {
+            printf("Wrote function type array successfully\n");
+        }
+        static const char *names[4] = {"offset", "amplitude", "period", "constant"};
+        static const double defaults[4] = {0., 1., 1., 1.};
+        static const char *read_ok[4] = {"Read offset values successfully\n", "Read amplitude values correctly\n",
+                                         "Read period values correctly\n", "Read constant values correctly\n"};
+        static const char *wrote_ok[4] = {"Wrote offset values sucessfully\n", "Wrote amplitude values correctly\n",
+                                          "Wrote period values correctly\n", "Wrote constant values correctly\n"};
+        for (int p = 0; p < 4; p++) {
+            if (tv_file[p + 1]) {
+                char msg[128];
+                snprintf(msg, sizeof msg, "ERROR: Issue while reading %s file", names[p]);
+                tv_par[p] = read_vector(tv_file[p + 1], n + 1, "%lf, ", msg);
+                if (!tv_par[p]) {
+                    printf("ERROR: Cannot open file containing %s values. Aborting\n", names[p]);
+                    exit(-1);
+                }
+                printf("%s", read_ok[p]);
+            } else {
+                tv_par[p] = (double *)malloc(((size_t)n + 1) * sizeof(double));
+                for (int k = 0; k <= n; k++) tv_par[p][k] = defaults[p];
+                printf("%s", wrote_ok[p]);
+            }
+        }
+        gpu_check(tasep_set_time_variant(ctx, tv_type, tv_par[1], tv_par[2], tv_par[0], tv_par[3]),
+                  "setting time-variant rates");
+        printf("Starting Calculation\n");
+    } else {
+        if (!rates) die("ERROR: no jump rates given (-i, --alpha/--beta or stdin)", -1);
+        gpu_check(tasep_set_rates(ctx, rates), "setting rates");
+    }
+    gpu_check(tasep_set_x0_uniform(ctx), "setting the initial state");
+
+    const double t_start = now_s();
+    if (fast) {
+        printf("Starting Calculation!\n");
+        printf("Generating Matrix:\n");
+        const double g0 = now_s();
+        gpu_check(tasep_build_csr(ctx), "building the generator");
+        printf("Generating Matrix: 100.000000%%\nTime elapsed: %f seconds\nCalculating result vector..\n\n",
+               now_s() - g0);
+    }
+    tasep_run_info info;
+    memset(&info, 0, sizeof info);
+    const double stop_condition = 1e-7;
+    snapshots_t snaps;
+    snapshots_init(&snaps, n, dim, have_a ? anim_res : 0, steps, save_p);
+    if (have_a && anim_res > 0) {
+        /* snapshots after steps 0, a, 2a, ... (src/masterEquation.c:646-666) */
+        uint64_t done = 0;
+        tasep_run_info part;
+        int first = 1;
+        while (done < steps) {
+            const uint64_t chunk = first ? 1 : (uint64_t)anim_res;
+            const uint64_t todo = done + chunk <= steps ? chunk : steps - done;
+            gpu_check(tasep_run_resume(ctx, t, h, todo, stop_condition, tvar ? -1 : 100, !first, &part),
+                      "running the time loop");
+            const uint64_t ran = part.steps_done - done;
+            done = part.steps_done;
+            info = part;
+            if (ran > 0 && (done - 1) % (uint64_t)anim_res == 0) snapshots_take(&snaps, ctx);
+            if (part.stopped || ran < todo) break; /* left early */
+            first = 0;
+        }
+    } else {
+        gpu_check(tasep_run(ctx, t, h, steps, stop_condition, tvar ? -1 : 100, &info), "running the time loop");
+    }
+    if (tvar) printf("\n");
+    const double secs = now_s() - t_start;
+    FILE *rt = fopen("runtimes.txt", "a");
+    if (rt) {
+        fprintf(rt, "%i %f\n", n, secs);
+        fclose(rt);
+    }
+    printf("\n####Finished Calculation####\n");
+    printf("Real Time elapsed: %f\n", secs);
+    printf("Simulation time elapsed: %lf\n", info.t);
+    if (info.early_stop)
+        printf("Convergence after %" PRIu64 " steps\nConfidience |Δx| = %.20g\n", info.steps_at_stop,
+               stop_condition * h);
+
+    /* ---- moments on the device, outputs on the host */
+    double *y = (double *)malloc((size_t)n * sizeof(double));
+    double *cov = (double *)malloc((size_t)n * n * sizeof(double));
+    double *mul = (double *)malloc((size_t)n * n * sizeof(double));
+    double mass = 0;
+    gpu_check(tasep_moments(ctx, y, cov, mul, &mass), "computing moments");
+    printf("Checksum over result vector: %lf\n", mass);
+
+    double *x = (double *)malloc(dim * sizeof(double));
+    if (!x) die("ERROR: Not enough memory to allocate x array", -1);
+    gpu_check(tasep_get_x(ctx, x), "reading the state vector");
+
+    out_ctx_t oc;
+    oc.n = n;
+    oc.dim = dim;
+    oc.h = h;
+    oc.anim_res = anim_res;
+    oc.o = o_file;
+    oc.save_e = save_e;
+    oc.save_p = save_p;
+    oc.save_c = save_c;
+    oc.save_m = save_m;
+    oc.have_a = have_a;
+    write_outputs(&oc, x, y, cov, mul, &snaps);
+
+    snapshots_free(&snaps);
+    tasep_ctx_destroy(ctx);
+    free(x); free(y); free(cov); free(mul); free(rates);
+    free(i_file); free(o_file); free(save_sel); free(tv_type);
+    for (int p = 0; p < 5; p++) free(tv_file[p]);
+    for (int p = 0; p < 4; p++) free(tv_par[p]);
+    return 0;
+}

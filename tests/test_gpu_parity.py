@@ -236,6 +236,71 @@ def test_converged_rk4_matches_dehp():
     assert np.max(np.abs(y[::-1] - rho)) < 1e-7
 
 
+# ----------------------------------------------------- tiled kernel coverage
+def _rk4_run(n, prec, r, steps, **opts):
+    with T.Engine(n, prec, "rk4", "mf") as e:
+        for k, v in opts.items():
+            e.set_option(getattr(T, "OPT_" + k.upper()), int(v))
+        e.set_rates(r)
+        e.set_x0_state(0)
+        e.run(0.0, 0.01, steps, patience=-1)
+        return e.get_x()
+
+
+@pytest.mark.parametrize("n", [12, 13, 14, 15, 17])
+def test_tiled_f64_matches_per_state_kernel(n):
+    r = np.random.default_rng(40 + n).uniform(0.2, 1.5, n + 1)
+    xt = _rk4_run(n, "f64", r, 60, tiled=1)
+    xs = _rk4_run(n, "f64", r, 60, tiled=0)
+    assert np.max(np.abs(xt - xs)) <= 1e-14
+
+
+@pytest.mark.parametrize("n", [13, 14, 15, 16, 18])
+def test_tiled_f32_matches_f64_oracle(n):
+    r = np.random.default_rng(50 + n).uniform(0.2, 1.5, n + 1)
+    x0 = np.zeros(1 << n)
+    x0[0] = 1.0
+    ox, _ = oracle.run_rk4(n, r, x0, 0.0, 0.01, 60)
+    xt = _rk4_run(n, "f32", r, 60, tiled=1)
+    err = np.max(np.abs(xt - ox))
+    print(f"n={n} FP32 tiled max|x - x_f64| = {err:.3e}")
+    assert err <= 1e-6
+    assert abs(xt.sum() - 1.0) <= 1e-5
+
+
+def test_xcd_order_does_not_change_results():
+    n = 17
+    r = np.random.default_rng(7).uniform(0.2, 1.5, n + 1)
+    a = _rk4_run(n, "f32", r, 30, xcd_order=1)
+    b = _rk4_run(n, "f32", r, 30, xcd_order=0)
+    c = _rk4_run(n, "f32", r, 30, tile_blocks=64)
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("prec,tiled", [("f64", 1), ("f64", 0), ("f32", 1)])
+def test_horner_equals_staged_rk4(prec, tiled):
+    n = 15
+    r = np.random.default_rng(8).uniform(0.2, 1.5, n + 1)
+    a = _rk4_run(n, prec, r, 50, tiled=tiled, rk4_staged=0)
+    b = _rk4_run(n, prec, r, 50, tiled=tiled, rk4_staged=1)
+    assert np.max(np.abs(a - b)) <= (1e-14 if prec == "f64" else 2e-7)
+
+
+def test_time_variant_tiled_vs_oracle():
+    n = 15
+    rng = np.random.default_rng(12)
+    tv = dict(type=np.zeros(n + 1, np.int32), amp=rng.uniform(0, 0.5, n + 1),
+              period=rng.uniform(0.5, 2, n + 1), offset=np.zeros(n + 1), c=np.ones(n + 1))
+    ox, _ = oracle.run_rk4(n, None, oracle.uniform_x0(n), 0.0, 0.01, 100, tv=tv)
+    for prec, tol in (("f64", 1e-12), ("f32", 2e-7)):
+        with T.Engine(n, prec, "rk4", "mf") as e:
+            e.set_time_variant(tv)
+            e.set_x0_uniform()
+            e.run(0.0, 0.01, 100)
+            x = e.get_x()
+        assert np.max(np.abs(x - ox)) <= tol, prec
+
+
 # ------------------------------------------------------------ error handling
 def test_errors_are_reported():
     with pytest.raises(T.TasepError):

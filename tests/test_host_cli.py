@@ -1,0 +1,125 @@
+"""The C host (tasep_amd/bin/masterEquation) against the reference binary's
+own outputs (tests/golden/cli/*): same command line, same files.
+
+CPU tests cover argument handling that ends before any GPU call; GPU tests
+run each golden case and compare every output file.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests import _golden as G
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(REPO, "tasep_amd", "bin", "masterEquation")
+
+needs_bin = pytest.mark.skipif(not os.path.exists(BIN), reason="host binary not built")
+
+
+def _run(args, cwd, stdin=""):
+    return subprocess.run([BIN] + args, cwd=cwd, input=stdin, capture_output=True,
+                          text=True, timeout=300)
+
+
+@needs_bin
+def test_fast_with_time_var_is_refused(tmp_path):
+    p = _run(["-n", "4", "-f", "--time-var", "-t", "0", "-T", "1", "-s", "10"], tmp_path)
+    assert p.returncode == 255
+    assert "-f cannot be selected in combination with --time-var" in p.stderr
+
+
+@needs_bin
+def test_missing_alpha_is_refused(tmp_path):
+    p = _run(["-n", "4", "--beta", "0.5", "-t", "0", "-T", "1", "-s", "10"], tmp_path)
+    assert p.returncode == 255
+    assert "No value for alpha found" in p.stderr
+
+
+@needs_bin
+def test_missing_beta_is_refused(tmp_path):
+    p = _run(["-n", "4", "--alpha", "0.5", "--gamma", "1.000000000000000000000000001",
+              "-t", "0", "-T", "1", "-s", "10"], tmp_path)
+    # the reference aborts here with a double free; the host reports the missing beta
+    assert p.returncode == 255
+    assert "No value for beta found" in p.stderr
+
+
+@needs_bin
+def test_unknown_option_prints_usage(tmp_path):
+    p = _run(["-x"], tmp_path)
+    assert p.returncode == 255 and "Usage:" in p.stdout
+
+
+# --------------------------------------------------------------- GPU runs
+CASES = G.cli_case_names()
+BYTE_EQUAL = ["out_exp_values_fin.vec", "out_exp_values_fin.dat", "out_cov.mat",
+              "out_cov.ppm", "out_mulExpVa", "out_error_guess", "out_closures",
+              "out_high_order_point_fun", "out_prb.txt"]
+
+
+def _stdout_line(text, prefix):
+    for chunk in text.replace("\r", "\n").split("\n"):
+        if chunk.startswith(prefix):
+            return chunk
+    return None
+
+
+@needs_bin
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_host_matches_reference_files(name, tmp_path):
+    c = G.cli_case(name)
+    for f in os.listdir(c["dir"]):
+        if f.endswith(".txt") and not f.startswith("out"):
+            shutil.copy(os.path.join(c["dir"], f), tmp_path / f)
+    print(f"case {name}: {' '.join(c['argv'])}", flush=True)
+    p = _run(c["argv"], tmp_path)
+    assert p.returncode == 0, p.stderr
+    produced = sorted(f for f in os.listdir(tmp_path) if f.startswith("out"))
+    expected = sorted(f for f in os.listdir(c["dir"]) if f.startswith("out"))
+    assert produced == expected
+    # probabilities: exact text for constant rates (bit-exact doubles)
+    got = open(tmp_path / "out").read()
+    ref = G.read_text(c, "out")
+    if c["tv"] is None:
+        assert got == ref
+    else:
+        gx = np.array([float(ln.split("\t\t")[1]) for ln in got.splitlines()])
+        assert np.max(np.abs(gx - c["x"])) <= 1e-12
+    for f in BYTE_EQUAL:
+        assert open(tmp_path / f, "rb").read() == G.read_bytes(c, f), f
+    # _highcorr: layout only (the reference prints an undefined value there)
+    a = open(tmp_path / "out_highcorr").read().splitlines()
+    b = G.read_text(c, "out_highcorr").splitlines()
+    assert a[0] == b[0] and [ln.split("\t")[0] for ln in a] == [ln.split("\t")[0] for ln in b]
+    for prefix in ("Convergence after", "Confidience", "Checksum over result vector",
+                   "Generated inital vector"):
+        want = G.stdout_value(c, prefix)
+        line = _stdout_line(p.stdout, prefix)
+        if want is None:
+            assert line is None
+        else:
+            assert line is not None and line[len(prefix):].strip() == want
+    sim = [ln for ln in p.stdout.splitlines() if ln.startswith("Simulation time elapsed: ")]
+    assert sim and sim[-1][len("Simulation time elapsed:"):].strip() == \
+        G.stdout_value(c, "Simulation time elapsed:")
+
+
+@needs_bin
+@pytest.mark.gpu
+def test_host_snapshots_option(tmp_path):
+    """-a: snapshot files exist with one row per snapshot (steps 0, a, 2a...)."""
+    (tmp_path / "rates.txt").write_text("0.4 1 1 1 1 0.6")
+    p = _run(["-n", "5", "-i", "rates.txt", "-t", "0", "-T", "1", "-s", "100", "-a", "10",
+              "-o", "out"], tmp_path, stdin="Y\nY\n")
+    assert p.returncode == 0, p.stderr
+    rows = open(tmp_path / "out_exp_timed.txt").read().splitlines()
+    assert len(rows) == 10
+    assert rows[0].startswith("0.000000, ") and rows[1].startswith("0.100000, ")
+    prb = open(tmp_path / "out_prb.txt").read().splitlines()
+    assert len(prb) == 10 and len(prb[0].split(", ")) == 1 + 32
+    assert os.path.isdir(tmp_path / "out_covariance_data")
+    assert os.path.exists(tmp_path / "out_highcorr_timed")
