@@ -176,6 +176,117 @@ def cpu_baseline(n, rates, h, budget_s=20.0):
                         f" + serial update), FP64, {threads} OpenMP threads, {el:.1f} s"))
 
 
+def step_h(T_end, t0, steps):
+    """h = (T - t)/(long double)steps, as the reference driver computes it."""
+    return float(np.longdouble(float(T_end) - float(t0)) / np.longdouble(steps))
+
+
+def timed_run(ev, e, t0, h, steps, **kw):
+    a, b = ev.event(), ev.event()
+    ev.device_sync()
+    ev.record(a, e.stream)
+    info = e.run(t0, h, steps, **kw)
+    ev.record(b, e.stream)
+    return info, ev.elapsed_ms(a, b)
+
+
+def config_1(T, ev):
+    """n=10, alpha=beta=1/2, gamma=1, empty lattice, t=0..100, 10000 steps,
+    reference loop in FP64, CSR and matrix-free."""
+    n = 10
+    r = np.array([0.5] + [1.0] * (n - 1) + [0.5])
+    h = step_h(100.0, 0.0, 10000)
+    res, xs = {}, {}
+    for gen in ("csr", "mf"):
+        with T.Engine(n, "f64", "ref", gen) as e:
+            e.set_rates(r)
+            e.set_x0_state(0)
+            info, ms = timed_run(ev, e, 0.0, h, 10000)
+            xs[gen] = e.get_x()
+            y, _, _, mass = e.moments()
+        res[gen] = {"ms": ms, "steps_done": info.steps_done, "stopped": info.stopped,
+                    "state_updates_per_s": info.steps_done * (1 << n) / (ms * 1e-3),
+                    "t": info.t, "mass": mass, "density_entry_to_exit": list(y[::-1])}
+    res["csr_equals_mf_bitwise"] = bool(np.array_equal(xs["csr"].view(np.uint64),
+                                                        xs["mf"].view(np.uint64)))
+    return res
+
+
+def config_2(T, ev):
+    """n=20, rates ~ U[0.1,2] (seed 20), uniform start, t=0..200, 20000 steps,
+    early exit; CSR vs matrix-free (reference loop) and RK4 FP64."""
+    n = 20
+    r = np.random.default_rng(20).uniform(0.1, 2.0, n + 1)
+    h = step_h(200.0, 0.0, 20000)
+    res, xs = {}, {}
+    for name, integ, gen, thr in (("ref_csr", "ref", "csr", 1e-7), ("ref_mf", "ref", "mf", 1e-7),
+                                  ("rk4_mf", "rk4", "mf", 1e-7), ("rk4_mf_thr1e-10", "rk4", "mf", 1e-10)):
+        with T.Engine(n, "f64", integ, gen) as e:
+            e.set_rates(r)
+            e.set_x0_uniform()
+            info, ms = timed_run(ev, e, 0.0, h, 20000, thr=thr, patience=100)
+            if name.startswith("ref"):
+                xs[name] = e.get_x()
+            y, _, _, mass = e.moments()
+        res[name] = {"ms": ms, "steps_done": info.steps_done, "stopped": info.stopped,
+                     "steps_at_stop": info.steps_at_stop,
+                     "state_updates_per_s": info.steps_done * (1 << n) / (ms * 1e-3),
+                     "t": info.t, "mass": mass}
+    res["ref_csr_equals_mf_bitwise"] = bool(np.array_equal(xs["ref_csr"].view(np.uint64),
+                                                            xs["ref_mf"].view(np.uint64)))
+    return res
+
+
+def config_3(T, ev):
+    """n=28 precision: FP32 state vs FP64, 5000 steps (t=0..50)."""
+    n, h, steps = 28, 0.01, 5000
+    r = workload_rates(n)
+    out = {}
+    mom = {}
+    for prec in ("f32", "f64"):
+        with T.Engine(n, prec, "rk4", "mf") as e:
+            e.set_rates(r)
+            e.set_x0_uniform()
+            info, ms = timed_run(ev, e, 0.0, h, steps, patience=-1)
+            y, cov, _, mass = e.moments()
+        mom[prec] = (y, cov)
+        out[prec] = {"ms": ms, "ms_per_step": ms / steps,
+                     "state_updates_per_s": steps * (1 << n) / (ms * 1e-3),
+                     "mass_before_normalisation": info.mass, "t": info.t}
+    out["max_density_err_f32_vs_f64"] = float(np.max(np.abs(mom["f32"][0] - mom["f64"][0])))
+    out["max_cov_err_f32_vs_f64"] = float(np.max(np.abs(mom["f32"][1] - mom["f64"][1])))
+    out["speedup_f32_over_f64"] = out["f32"]["state_updates_per_s"] / out["f64"]["state_updates_per_s"]
+    return out
+
+
+def config_5(T, ev):
+    """n=32: alpha=0.3, beta=0.7, gamma=1 from the empty lattice, h=0.01;
+    FP32 1000 steps (matrix-free), drift against an FP64 run at step 100."""
+    n, h = 32, 0.01
+    r = np.array([0.3] + [1.0] * (n - 1) + [0.7])
+    out = {}
+    with T.Engine(n, "f32", "rk4", "mf") as e:
+        e.set_rates(r)
+        e.set_x0_state(0)
+        info, ms100 = timed_run(ev, e, 0.0, h, 100, patience=-1)
+        y32_100, _, _, _ = e.moments()
+        info, ms = timed_run(ev, e, 0.0, h, 900, patience=-1, resume=True)
+        y32, cov32, _, mass = e.moments()
+        out["f32"] = {"ms_1000_steps": ms100 + ms, "ms_per_step": (ms100 + ms) / 1000,
+                      "state_updates_per_s": 1000 * (1 << n) / ((ms100 + ms) * 1e-3),
+                      "t": info.t, "steps": info.steps_done, "mass": mass,
+                      "density_entry_to_exit_t10": list(y32[::-1])}
+    with T.Engine(n, "f64", "rk4", "mf") as e:
+        e.set_rates(r)
+        e.set_x0_state(0)
+        info, ms = timed_run(ev, e, 0.0, h, 100, patience=-1)
+        y64_100, _, _, _ = e.moments()
+        out["f64"] = {"ms_100_steps": ms, "ms_per_step": ms / 100,
+                      "state_updates_per_s": 100 * (1 << n) / (ms * 1e-3)}
+    out["max_density_drift_f32_vs_f64_step100"] = float(np.max(np.abs(y32_100 - y64_100)))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +296,8 @@ def main():
     ap.add_argument("--h", type=float, default=0.01)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
+    ap.add_argument("--config", type=int, default=0,
+                    help="run one of the five baseline configurations at full size instead")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -194,6 +307,12 @@ def main():
     import tasep_amd as T
     T.load_library()
     ev = HipEvents()
+    if args.config:
+        fn = {1: config_1, 2: config_2, 3: config_3, 5: config_5}.get(args.config)
+        if fn is None:
+            sys.exit(f"configuration {args.config} is not available")
+        print(json.dumps({"config": args.config, "result": fn(T, ev)}), flush=True)
+        return
     n, h, K, W = args.n, args.h, args.steps, args.warmup
     rates = workload_rates(n)
     dim = 1 << n
@@ -216,7 +335,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded rates, uniform initial state)",
         "config": {"workload": f"n{n}_f32_rk4_mf", "n": n, "states": dim, "h": h,
-                   "rk4_form": "horner", "storage": "f32", "arithmetic": "f64",
+                   "rk4_form": "horner", "storage": "f32",
+                   "arithmetic": "f32 generator, f64 checksum/mass/moments",
                    "rates": "alpha,beta~U[0.2,1], bulk~LogNormal(0,0.3), seed 28",
                    "wall_s": wall32},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved,

@@ -345,13 +345,17 @@ __global__ __launch_bounds__(kBlock) void k_moments_partial(const V *__restrict_
     }
 }
 
-__global__ void k_moments_final(const double *__restrict__ part, int nblocks, int nq,
-                                double *__restrict__ out) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
+// One workgroup per quantity: lane t sums partials t, t+256, ... (Neumaier),
+// then a fixed-shape tree; deterministic for a given partial count.
+__global__ __launch_bounds__(kBlock) void k_moments_final(const double *__restrict__ part,
+                                                          int nblocks, int nq,
+                                                          double *__restrict__ out) {
+    __shared__ double red[kWaves];
+    const int q = blockIdx.x;
     Kahan k;
-    for (int b = 0; b < nblocks; b++) k.add(part[(uint64_t)b * nq + q]);
-    out[q] = k.value();
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) k.add(part[(uint64_t)b * nq + q]);
+    const double v = block_sum(k.value(), red);
+    if (threadIdx.x == 0) out[q] = v;
 }
 
 // ===========================================================================

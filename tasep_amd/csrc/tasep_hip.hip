@@ -767,8 +767,8 @@ int tasep_moments(tasep_ctx *c, double *y, double *cov, double *mul, double *mas
     else
         hipLaunchKernelGGL(k_moments_partial<double>, dim3(nb), dim3(kBlock), 0, c->stream,
                            (const double *)xcur(c), n, c->dim, rows, c->mom_part);
-    hipLaunchKernelGGL(k_moments_final, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
-                       c->mom_part, nb, nq, c->mom_out);
+    hipLaunchKernelGGL(k_moments_final, dim3(nq), dim3(kBlock), 0, c->stream, c->mom_part, nb, nq,
+                       c->mom_out);
     HIP_OK(hipGetLastError());
     std::vector<double> E(nq);
     HIP_OK(hipMemcpyAsync(E.data(), c->mom_out, nq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
