@@ -197,18 +197,22 @@ def config_1(T, ev):
     r = np.array([0.5] + [1.0] * (n - 1) + [0.5])
     h = step_h(100.0, 0.0, 10000)
     res, xs = {}, {}
-    for gen in ("csr", "mf"):
+    # mf: the LDS-resident whole-run kernel (default for n <= 13);
+    # mf_streaming: one launch per step over HBM
+    for name, gen, lds in (("csr", "csr", 1), ("mf", "mf", 1), ("mf_streaming", "mf", 0)):
         with T.Engine(n, "f64", "ref", gen) as e:
+            e.set_option(T.OPT_LDS, lds)
             e.set_rates(r)
             e.set_x0_state(0)
             info, ms = timed_run(ev, e, 0.0, h, 10000)
-            xs[gen] = e.get_x()
+            xs[name] = e.get_x()
             y, _, _, mass = e.moments()
-        res[gen] = {"ms": ms, "steps_done": info.steps_done, "stopped": info.stopped,
-                    "state_updates_per_s": info.steps_done * (1 << n) / (ms * 1e-3),
-                    "t": info.t, "mass": mass, "density_entry_to_exit": list(y[::-1])}
-    res["csr_equals_mf_bitwise"] = bool(np.array_equal(xs["csr"].view(np.uint64),
-                                                        xs["mf"].view(np.uint64)))
+        res[name] = {"ms": ms, "steps_done": info.steps_done, "stopped": info.stopped,
+                     "state_updates_per_s": info.steps_done * (1 << n) / (ms * 1e-3),
+                     "t": info.t, "mass": mass, "density_entry_to_exit": list(y[::-1])}
+    res["csr_equals_mf_bitwise"] = all(
+        bool(np.array_equal(xs["csr"].view(np.uint64), xs[k].view(np.uint64)))
+        for k in ("mf", "mf_streaming"))
     return res
 
 

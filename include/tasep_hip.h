@@ -157,11 +157,21 @@ int tasep_step_profile(tasep_ctx *ctx, double t0, double h, int reps,
  *                              rates (default: the Horner form of the same
  *                              step, 3 work vectors instead of 5)
  *   TASEP_OPT_TILE_BLOCKS  [0] workgroups of the tiled passes (multiple of
- *                              8); 0 = as many as are resident at once */
+ *                              8); 0 = as many as are resident at once
+ *   TASEP_OPT_LDS          [1] matrix-free runs that keep two copies of the
+ *                              state vector in one workgroup's LDS and run
+ *                              all their steps there, in one launch per
+ *                              ~2^27 state-updates (<= 65536 steps):
+ *                              0 never, 1 for n <= 12, 2 wherever it fits
+ *                              (n <= 13)
+ *   TASEP_OPT_LDS_THREADS  [0] workgroup size of those runs (power of two in
+ *                              [64, 1024]); 0 = 2^n clamped to that range */
 #define TASEP_OPT_TILED        1
 #define TASEP_OPT_XCD_ORDER    2
 #define TASEP_OPT_RK4_STAGED   3
 #define TASEP_OPT_TILE_BLOCKS  4
+#define TASEP_OPT_LDS          5
+#define TASEP_OPT_LDS_THREADS  6
 int tasep_set_option(tasep_ctx *ctx, int key, int64_t value);
 
 /* The context's HIP stream (hipStream_t) and device pointer of the current
@@ -169,6 +179,53 @@ int tasep_set_option(tasep_ctx *ctx, int key, int64_t value);
  * consume the state on the device. */
 int tasep_get_stream(tasep_ctx *ctx, void **stream);
 int tasep_get_x_device(tasep_ctx *ctx, void **ptr);
+
+/* ------------------------------------------------------------------------
+ * Batches: many independent instances (own n, rates or rate functions,
+ * state and run state) driven together.  No reference counterpart: the
+ * reference runs one masterEquation process per instance
+ * (src/masterEquation.c:main); a batch replaces a loop over such runs with
+ * classical RK4 (time-variant instances with stored stages).  Instances
+ * with n <= 13 run all their steps inside one workgroup each (one launch
+ * for all of them); larger ones share one launch per RK4 pass, replayed
+ * from a captured HIP graph.  Run info and early exit are per instance.
+ * ---------------------------------------------------------------------- */
+typedef struct tasep_batch tasep_batch;
+
+/* count instances of n[i] sites (2 <= n[i] <= 30), TASEP_F32 or TASEP_F64;
+ * every state starts uniform. */
+int tasep_batch_create(tasep_batch **batch, int count, const int32_t *n,
+                       int precision);
+int tasep_batch_destroy(tasep_batch *batch);
+/* Rates of instance i (n[i]+1 values, as tasep_set_rates) or its rate
+ * functions (as tasep_set_time_variant). */
+int tasep_batch_set_rates(tasep_batch *batch, int i, const double *r);
+int tasep_batch_set_time_variant(tasep_batch *batch, int i,
+                                 const int32_t *type, const double *amp,
+                                 const double *period, const double *offset,
+                                 const double *c);
+int tasep_batch_set_x0_uniform(tasep_batch *batch);
+int tasep_batch_set_x(tasep_batch *batch, int i, const double *x);
+int tasep_batch_get_x(tasep_batch *batch, int i, double *x);
+/* Run every instance `steps` steps from t0; info (may be NULL) receives
+ * `count` entries.  thr/patience as tasep_run, per instance. */
+int tasep_batch_run(tasep_batch *batch, double t0, double h, uint64_t steps,
+                    double thr, int patience, tasep_run_info *info);
+/* Moments of instance i, as tasep_moments. */
+int tasep_batch_moments(tasep_batch *batch, int i, double *y, double *cov,
+                        double *mul, double *mass);
+/* Batch options (defaults in brackets):
+ *   TASEP_BATCH_OPT_GRAPH_STEPS [32] steps per captured HIP graph of the
+ *                                    shared passes; 0 = direct launches
+ *   TASEP_BATCH_OPT_LDS         [1]  instances with n <= 13 run in LDS;
+ *                                    0 = only those with n < 10 do
+ *   TASEP_BATCH_OPT_CONCURRENT  [1]  the LDS-resident instances run on a
+ *                                    second stream beside the shared passes */
+#define TASEP_BATCH_OPT_GRAPH_STEPS 1
+#define TASEP_BATCH_OPT_LDS         2
+#define TASEP_BATCH_OPT_CONCURRENT  3
+int tasep_batch_set_option(tasep_batch *batch, int key, int64_t value);
+int tasep_batch_get_stream(tasep_batch *batch, void **stream);
 
 const char *tasep_strerror(int code);
 

@@ -31,8 +31,13 @@ EXPORTS = [
     "tasep_csr_nnz", "tasep_gencrs", "tasep_run", "tasep_moments",
     "tasep_get_stream", "tasep_get_x_device", "tasep_strerror",
     "tasep_step_profile", "tasep_set_option", "tasep_run_resume",
+    "tasep_batch_create", "tasep_batch_destroy", "tasep_batch_set_rates",
+    "tasep_batch_set_time_variant", "tasep_batch_set_x0_uniform", "tasep_batch_set_x",
+    "tasep_batch_get_x", "tasep_batch_run", "tasep_batch_moments",
+    "tasep_batch_set_option", "tasep_batch_get_stream",
 ]
-OPT_TILED, OPT_XCD_ORDER, OPT_RK4_STAGED, OPT_TILE_BLOCKS = 1, 2, 3, 4
+OPT_TILED, OPT_XCD_ORDER, OPT_RK4_STAGED, OPT_TILE_BLOCKS, OPT_LDS, OPT_LDS_THREADS = 1, 2, 3, 4, 5, 6
+BATCH_OPT_GRAPH_STEPS, BATCH_OPT_LDS, BATCH_OPT_CONCURRENT = 1, 2, 3
 
 
 class TasepError(RuntimeError):
@@ -106,6 +111,18 @@ def load_library():
     L.tasep_set_option.argtypes = [vp, C.c_int, C.c_int64]
     L.tasep_get_stream.argtypes = [vp, C.POINTER(vp)]
     L.tasep_get_x_device.argtypes = [vp, C.POINTER(vp)]
+    L.tasep_batch_create.argtypes = [C.POINTER(vp), C.c_int, _i32p, C.c_int]
+    L.tasep_batch_destroy.argtypes = [vp]
+    L.tasep_batch_set_rates.argtypes = [vp, C.c_int, _f64p]
+    L.tasep_batch_set_time_variant.argtypes = [vp, C.c_int, _i32p, _f64p, _f64p, _f64p, _f64p]
+    L.tasep_batch_set_x0_uniform.argtypes = [vp]
+    L.tasep_batch_set_x.argtypes = [vp, C.c_int, _f64p]
+    L.tasep_batch_get_x.argtypes = [vp, C.c_int, _f64p]
+    L.tasep_batch_run.argtypes = [vp, C.c_double, C.c_double, C.c_uint64, C.c_double,
+                                  C.c_int, C.POINTER(_RunInfo)]
+    L.tasep_batch_moments.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+    L.tasep_batch_set_option.argtypes = [vp, C.c_int, C.c_int64]
+    L.tasep_batch_get_stream.argtypes = [vp, C.POINTER(vp)]
     for name in EXPORTS:
         if name not in ("tasep_strerror", "tasep_csr_nnz"):
             getattr(L, name).restype = C.c_int
@@ -252,4 +269,90 @@ class Engine:
     def x_device_ptr(self) -> int:
         p = C.c_void_p()
         _check(load_library().tasep_get_x_device(self._h, C.byref(p)))
+        return int(p.value or 0)
+
+
+def _info(ri) -> RunInfo:
+    return RunInfo(int(ri.steps_done), int(ri.steps_at_stop), int(ri.early_stop),
+                   bool(ri.stopped), float(ri.t), float(ri.last_checksum), float(ri.mass))
+
+
+class Batch:
+    """Many independent instances (own n and rates) driven together."""
+
+    def __init__(self, ns, precision="f32"):
+        L = load_library()
+        self.ns = [int(v) for v in ns]
+        self.count = len(self.ns)
+        self.precision = _PREC[precision]
+        h = C.c_void_p()
+        _check(L.tasep_batch_create(C.byref(h), self.count,
+                                    np.ascontiguousarray(self.ns, np.int32), self.precision))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load_library().tasep_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_rates(self, i, r):
+        r = np.ascontiguousarray(r, np.float64)
+        if 0 <= i < self.count and r.shape != (self.ns[i] + 1,):
+            raise ValueError(f"instance {i} needs {self.ns[i] + 1} rates")
+        _check(load_library().tasep_batch_set_rates(self._h, i, r))
+
+    def set_time_variant(self, i, tv):
+        f = lambda k: np.ascontiguousarray(tv[k], np.float64)  # noqa: E731
+        _check(load_library().tasep_batch_set_time_variant(
+            self._h, i, np.ascontiguousarray(tv["type"], np.int32), f("amp"), f("period"),
+            f("offset"), f("c")))
+
+    def set_x0_uniform(self):
+        _check(load_library().tasep_batch_set_x0_uniform(self._h))
+
+    def set_x(self, i, x):
+        x = np.ascontiguousarray(x, np.float64)
+        if x.shape != (1 << self.ns[i],):
+            raise ValueError("x must have 2^n entries")
+        _check(load_library().tasep_batch_set_x(self._h, i, x))
+
+    def get_x(self, i):
+        x = np.empty(1 << self.ns[i], np.float64)
+        _check(load_library().tasep_batch_get_x(self._h, i, x))
+        return x
+
+    def set_option(self, key: int, value: int):
+        _check(load_library().tasep_batch_set_option(self._h, key, value))
+
+    def run(self, t0, h, steps, thr=1e-7, patience=-1):
+        arr = (_RunInfo * self.count)()
+        _check(load_library().tasep_batch_run(self._h, t0, h, steps, thr, patience, arr))
+        return [_info(arr[i]) for i in range(self.count)]
+
+    def moments(self, i):
+        n = self.ns[i]
+        y = np.zeros(n)
+        cov = np.zeros(n * n)
+        mul = np.zeros(n * n)
+        m = np.zeros(1)
+        _check(load_library().tasep_batch_moments(self._h, i, y.ctypes.data, cov.ctypes.data,
+                                                  mul.ctypes.data, m.ctypes.data))
+        return y, cov.reshape(n, n), mul.reshape(n, n), float(m[0])
+
+    @property
+    def stream(self) -> int:
+        p = C.c_void_p()
+        _check(load_library().tasep_batch_get_stream(self._h, C.byref(p)))
         return int(p.value or 0)

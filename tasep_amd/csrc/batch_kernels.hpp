@@ -1,0 +1,189 @@
+// batch_kernels.hpp -- one RK4 pass over many instances in one launch.
+//
+// The instances of a batch that do not fit one workgroup's LDS (n >= 14)
+// share every launch: their vectors are concatenated in HBM and a block
+// table maps each workgroup to (instance, block of that instance).  A
+// workgroup of kBatchThreads lanes covers kBatchThreads * E consecutive
+// states of one instance; lane t holds states base + j*kBatchThreads + t, so
+// a wave holds 64 consecutive states and
+//   - exit and bonds k < 5 pull from inside the wave (cross-lane swizzles),
+//   - bond 5 pulls from the neighbouring 64-state window (one coalesced load),
+//   - bonds k >= 6 and the entry bit are the same for the whole wave, so
+//     they are scalar branches and only real in-bonds load anything.
+// Constant-rate instances take RK4 in Horner form (passes x->u, u->v, v->u,
+// u->x), time-variant ones RK4 with stored stages (accumulator vector acc,
+// rates evaluated in FP64 per block at the pass's time).  The last pass
+// reduces sum|dx| and sum x per instance and advances each instance's run
+// state (t, early exit) in its last block, so a captured sequence of passes
+// replays without host work.
+#pragma once
+
+#include "device_common.hpp"
+#include "lds_kernels.hpp"
+
+namespace tasep {
+
+constexpr int kBatchThreads = 256;
+constexpr int kBatchMaxN = 30;
+
+template <class V> struct BatchGeom;
+template <> struct BatchGeom<float> { static constexpr int E = 4; };
+template <> struct BatchGeom<double> { static constexpr int E = 2; };
+
+struct BatchInst {
+    uint64_t off;       // element offset of the instance in x, u, v, acc
+    int32_t n, tv;
+    int32_t rate_off;   // rates[rate_off .. rate_off+n]
+    int32_t tv_off;     // tv_*[tv_off .. tv_off+n]
+    uint32_t part_off;  // first per-block partial slot of the instance
+    uint32_t nblocks;   // workgroups of one pass over the instance
+    int32_t id;         // index of the instance's run state in BatchArgs::st
+};
+
+struct BatchBlock {
+    uint32_t inst;  // 0xffffffff: padding, no work
+    uint32_t blk;
+};
+
+template <class V>
+struct BatchArgs {
+    const BatchInst *inst;
+    const BatchBlock *blocks;  // one entry per workgroup
+    V *x, *u, *v, *acc;
+    const double *rates;
+    const int32_t *tv_type;
+    const double *tv_amp, *tv_period, *tv_offset, *tv_c;
+    RunState *st;              // run states, indexed by BatchInst::id
+    double *pchk, *pmass;      // per-block partials
+    int pass;                  // 0..3
+};
+
+// Rates of one pass in registers: the entry/exit rates and bonds k < 6 per
+// lane; bonds k >= 6 are read from LDS inside their (uniform) branch.
+template <class A>
+struct BatchRates {
+    A alpha, beta, bk[kLaneBits];
+};
+
+template <class V, class A>
+__device__ __forceinline__ A batch_diag(int n, uint32_t s, uint32_t hi, const BatchRates<A> &q, const V *rs) {
+    A d = (s & 1) ? q.beta : (A)0;
+    if (!(hi & (1u << (n - 1)))) d += q.alpha;
+#pragma unroll
+    for (int k = 0; k < kLaneBits; k++)
+        if (((s >> k) & 3u) == 2u) d += q.bk[k];
+    for (int k = kLaneBits; k < n - 1; k++)
+        if (((hi >> k) & 3u) == 2u) d += (A)rs[n - 1 - k];
+    return d;
+}
+
+// (L g)_s: own = g[s], g = the instance's gathered vector.
+template <class V, class A>
+__device__ __forceinline__ A batch_row(int n, uint32_t s, uint32_t hi, const V *__restrict__ g, A own,
+                                       const BatchRates<A> &q, const V *rs, A dpos) {
+    A acc = -dpos * own;
+    const uint32_t top = 1u << (n - 1);
+    if (hi & top) acc = fma(q.alpha, (A)g[s - top], acc);
+    acc = fma((s & 1) ? (A)0 : q.beta, (A)__shfl_xor(own, 1, kWave), acc);
+#pragma unroll
+    for (int k = 0; k < kLaneBits - 1; k++)
+        acc = fma(((s >> k) & 3u) == 1u ? q.bk[k] : (A)0, (A)__shfl_xor(own, 3 << k, kWave), acc);
+    // bond 5: bits 5 (per lane) and 6 (per wave); s ^ (3 << 5) is in the
+    // neighbouring window and inside [0, 2^n) for every lane
+    acc = fma(((s >> 5) & 3u) == 1u ? q.bk[5] : (A)0, (A)g[s ^ (3u << 5)], acc);
+    for (int k = kLaneBits; k < n - 1; k++)
+        if (((hi >> k) & 3u) == 1u) acc = fma((A)rs[n - 1 - k], (A)g[s + (1u << k)], acc);
+    return acc;
+}
+
+template <class V>
+__global__ __launch_bounds__(kBatchThreads) void k_batch_pass(const BatchArgs<V> a) {
+    using A = V;  // FP32 instances compute in FP32, FP64 ones in FP64
+    constexpr int E = BatchGeom<V>::E;
+    constexpr uint32_t B = kBatchThreads * E;
+    __shared__ V rs[kRateStride];
+    __shared__ double red[kBatchThreads / kWave];
+    const BatchBlock bb = a.blocks[blockIdx.x];
+    if (bb.inst == 0xffffffffu) return;
+    const BatchInst in = a.inst[bb.inst];
+    RunState *st = a.st + in.id;
+    if (st->stopped) return;
+    const int n = in.n, p = a.pass, t = threadIdx.x;
+    const double h = st->h;
+    // rates of this pass: constant, or r(t), r(t+h/2), r(t+h) for stages 1, 2-3, 4
+    if (t <= n) {
+        double r;
+        if (in.tv) {
+            const int o = in.tv_off + t;
+            const double t0 = st->t;
+            const double tq = p == 0 ? t0 : (p < 3 ? t0 + 0.5 * h : t0 + h);
+            r = tv_rate(a.tv_type[o], a.tv_amp[o], a.tv_period[o], a.tv_offset[o], a.tv_c[o], tq);
+        } else {
+            r = a.rates[in.rate_off + t];
+        }
+        rs[t] = (V)r;
+    }
+    __syncthreads();
+    BatchRates<A> q;
+    q.alpha = (A)rs[0];
+    q.beta = (A)rs[n];
+#pragma unroll
+    for (int k = 0; k < kLaneBits; k++) q.bk[k] = (A)rs[n - 1 - k];
+    V *x = a.x + in.off, *u = a.u + in.off, *v = a.v + in.off, *kacc = a.acc + in.off;
+    const V *g = p == 0 ? x : (p == 2 ? v : u);
+    V *dst = p == 1 ? v : u;
+    const uint32_t base = bb.blk * B;
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(base + (t & ~(kWave - 1))));
+    Kahan ck, km;
+    if (!in.tv) {
+        const A cs = p == 0 ? (A)(0.25 * h) : (p == 1 ? (A)(h / 3.0) : (p == 2 ? (A)(0.5 * h) : (A)h));
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const uint32_t s = base + j * kBatchThreads + t;
+            const uint32_t hj = hi + j * kBatchThreads;
+            const A own = (A)g[s];
+            const A lg = batch_row<V, A>(n, s, hj, g, own, q, rs, batch_diag<V, A>(n, s, hj, q, rs));
+            const A xs = p == 0 ? own : (A)x[s];
+            if (p < 3) {
+                dst[s] = (V)fma(cs, lg, xs);
+            } else {
+                const A d = cs * lg;
+                const V xn = (V)(xs + d);
+                x[s] = xn;
+                ck.add(fabs((double)d));
+                km.add((double)xn);
+            }
+        }
+    } else {
+        const A hv = (A)h;
+        const A cy = p == 2 ? (A)1 : (A)0.5;
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const uint32_t s = base + j * kBatchThreads + t;
+            const uint32_t hj = hi + j * kBatchThreads;
+            const A own = (A)g[s];
+            const A kk = hv * batch_row<V, A>(n, s, hj, g, own, q, rs, batch_diag<V, A>(n, s, hj, q, rs));
+            const A xs = p == 0 ? own : (A)x[s];
+            if (p == 0) {
+                kacc[s] = (V)kk;
+                dst[s] = (V)fma(cy, kk, xs);
+            } else if (p < 3) {
+                kacc[s] = (V)((A)kacc[s] + (A)2 * kk);
+                dst[s] = (V)fma(cy, kk, xs);
+            } else {
+                const A d = ((A)kacc[s] + kk) / (A)6;
+                const V xn = (V)(xs + d);
+                x[s] = xn;
+                ck.add(fabs((double)d));
+                km.add((double)xn);
+            }
+        }
+    }
+    if (p == 3) {
+        const double bc = block_sum<kBatchThreads>(ck.value(), red);
+        const double bm = block_sum<kBatchThreads>(km.value(), red);
+        finish_group<kBatchThreads>(st, a.pchk + in.part_off, a.pmass + in.part_off, bb.blk, in.nblocks, bc, bm);
+    }
+}
+
+}  // namespace tasep

@@ -72,29 +72,30 @@ __device__ __forceinline__ double load_relaxed(const double *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// End-of-step bookkeeping, called by every block of a step's final kernel
-// with its block partials of sum|dx| and sum x.  The last block to arrive
-// reduces all partials in a fixed order (so the result does not depend on
-// which block is last) and advances the run state exactly as the reference
-// loop does (src/masterEquation.c:644,670-679): t += h, then the
+// End-of-step bookkeeping, called by every block of a group of `nslots`
+// blocks (one instance's blocks of a step's final kernel) with its block
+// partials of sum|dx| and sum x, stored at slot `slot`.  The last block to
+// arrive reduces all partials in a fixed order (so the result does not
+// depend on which block is last) and advances the run state exactly as the
+// reference loop does (src/masterEquation.c:644,670-679): t += h, then the
 // sub-threshold counter and the exit test.
 template <int NT = kBlock>
-__device__ void finish_step(RunState *st, double *part_chk, double *part_mass,
-                            double blk_chk, double blk_mass) {
+__device__ void finish_group(RunState *st, double *part_chk, double *part_mass, uint32_t slot, uint32_t nslots,
+                             double blk_chk, double blk_mass) {
     __shared__ int s_last;
     __shared__ double s_red[NT / kWave];
     if (threadIdx.x == 0) {
-        part_chk[blockIdx.x] = blk_chk;
-        part_mass[blockIdx.x] = blk_mass;
+        part_chk[slot] = blk_chk;
+        part_mass[slot] = blk_mass;
         __threadfence();
         uint32_t prev = atomicAdd(&st->counter, 1u);
-        s_last = (prev == gridDim.x - 1);
+        s_last = (prev == nslots - 1);
     }
     __syncthreads();
     if (!s_last) return;
     __threadfence();
     Kahan kc, km;
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += NT) {
+    for (uint32_t b = threadIdx.x; b < nslots; b += NT) {
         kc.add(load_relaxed(part_chk + b));
         km.add(load_relaxed(part_mass + b));
     }
@@ -119,6 +120,12 @@ __device__ void finish_step(RunState *st, double *part_chk, double *part_mass,
         st->counter = 0u;
         __threadfence();
     }
+}
+
+// finish_group over the whole grid of the step's final kernel.
+template <int NT = kBlock>
+__device__ void finish_step(RunState *st, double *part_chk, double *part_mass, double blk_chk, double blk_mass) {
+    finish_group<NT>(st, part_chk, part_mass, blockIdx.x, gridDim.x, blk_chk, blk_mass);
 }
 
 // Stage the rate set `set` (n+1 doubles) into LDS.

@@ -15,54 +15,13 @@
 #include <vector>
 
 #include "../../include/tasep_hip.h"
+#include "host_common.hpp"
 #include "kernels.hpp"
+#include "lds_launch.hpp"
 #include "tile_kernels.hpp"
 
 using namespace tasep;
-
-namespace {
-
-constexpr int kMaxBlocks = 8192;       // grid cap of the streaming kernels
-constexpr int kMomMaxBlocks = 2048;
-
-thread_local char g_errmsg[256] = "";
-
-int fail(int code, const char *what, hipError_t e = hipSuccess) {
-    if (e != hipSuccess)
-        snprintf(g_errmsg, sizeof g_errmsg, "%s: %s", what, hipGetErrorString(e));
-    else
-        snprintf(g_errmsg, sizeof g_errmsg, "%s", what);
-    return code;
-}
-
-#define HIP_OK(expr)                                                        \
-    do {                                                                    \
-        hipError_t e_ = (expr);                                             \
-        if (e_ != hipSuccess)                                               \
-            return fail(e_ == hipErrorOutOfMemory ? TASEP_ERR_NOMEM         \
-                                                  : TASEP_ERR_HIP,          \
-                        #expr, e_);                                         \
-    } while (0)
-
-int check_device() {
-    int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
-        return fail(TASEP_ERR_NODEV, "no HIP device visible");
-    int dev = 0;
-    HIP_OK(hipGetDevice(&dev));
-    hipDeviceProp_t p;
-    HIP_OK(hipGetDeviceProperties(&p, dev));
-    if (strncmp(p.gcnArchName, "gfx950", 6) != 0)
-        return fail(TASEP_ERR_NODEV, "device is not gfx950 (MI355X)");
-    return TASEP_OK;
-}
-
-uint64_t grid_for(uint64_t dim, int cap = kMaxBlocks) {
-    uint64_t b = (dim + kBlock - 1) / kBlock;
-    return b < (uint64_t)cap ? (b ? b : 1) : (uint64_t)cap;
-}
-
-}  // namespace
+using namespace tasep_host;
 
 struct tasep_ctx {
     int n = 0, prec = 0, integ = 0, gen = 0;
@@ -89,7 +48,11 @@ struct tasep_ctx {
     bool xcd_order = true;      // XCD-aware tile order
     bool force_staged = false;  // staged RK4 even for constant rates
     int tile_blocks = 0;        // persistent grid of the tiled passes (0: by occupancy)
+    int lds = 1;                // whole runs in one workgroup's LDS: 0 never, 1 n <= 12, 2 where it fits
+    int lds_threads = 0;        // workgroup size of the LDS-resident kernel (0: automatic)
     uint64_t steps_before = 0;  // cumulative step count when the current run call began
+    LdsInstance *lds_inst = nullptr;       // device: the instance table of one entry
+    LdsInstance *lds_inst_host = nullptr;  // pinned
 };
 
 // --------------------------------------------------------------- utilities
@@ -142,6 +105,8 @@ int tasep_ctx_destroy(tasep_ctx *c) {
     hipFree(c->pmass);
     hipFree(c->mom_part);
     hipFree(c->mom_out);
+    hipFree(c->lds_inst);
+    if (c->lds_inst_host) hipHostFree(c->lds_inst_host);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return TASEP_OK;
@@ -189,6 +154,9 @@ int tasep_ctx_create(tasep_ctx **out, int n, int precision, int integrator, int 
              "hipHostMalloc(state)"));
     CK(hipck(hipMalloc(&c->pchk, kMaxBlocks * sizeof(double)), "hipMalloc(partials)"));
     CK(hipck(hipMalloc(&c->pmass, kMaxBlocks * sizeof(double)), "hipMalloc(partials)"));
+    CK(hipck(hipMalloc(&c->lds_inst, sizeof(LdsInstance)), "hipMalloc(instance)"));
+    CK(hipck(hipHostMalloc((void **)&c->lds_inst_host, sizeof(LdsInstance), hipHostMallocDefault),
+             "hipHostMalloc(instance)"));
     CK(ensure_buf(c, 0));
     memset(c->st_host, 0, sizeof(RunState));
     c->st_host->scale = 1.0;
@@ -568,6 +536,19 @@ static void ref_step(tasep_ctx *c, double h, uint64_t j) {
     }
 }
 
+static int lds_threads(const tasep_ctx *c) {
+    return c->lds_threads ? c->lds_threads : lds_auto_threads(c->n);
+}
+
+// A single instance runs LDS-resident by default up to n = 12: beyond that
+// one CU is slower than the streaming passes spread over the whole chip.
+constexpr int kLdsAutoMaxN = 12;
+
+static bool use_lds(const tasep_ctx *c) {
+    if (c->gen != TASEP_MF || !c->lds || !lds_fits(c->n, (int)c->esize, c->integ)) return false;
+    return c->lds == 2 || c->n <= kLdsAutoMaxN;
+}
+
 // Allocate what a run needs and reset the device loop state (resume: keep
 // t, the step count and the early-exit counters of the previous run).
 static int prepare_run(tasep_ctx *c, double t0, double h, double thr, int patience, int resume = 0) {
@@ -651,12 +632,13 @@ static void enqueue_step(tasep_ctx *c, double h, uint64_t j) {
 }
 
 // Read the loop state back and move the ping-pong index.
-static int finish_run(tasep_ctx *c, tasep_run_info *info) {
+static int finish_run(tasep_ctx *c, tasep_run_info *info, bool inplace = false) {
     HIP_OK(hipMemcpyAsync(c->st_host, c->st, sizeof(RunState), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     const RunState &s = *c->st_host;
     const bool staged = c->integ == TASEP_RK4 && (c->tv || c->force_staged);
-    if (c->integ == TASEP_REF || staged) c->cur ^= (int)((s.steps_done - c->steps_before) & 1);
+    if (!inplace && (c->integ == TASEP_REF || staged))
+        c->cur ^= (int)((s.steps_done - c->steps_before) & 1);
     c->steps_before = s.steps_done;
     if (info) {
         info->steps_done = s.steps_done;
@@ -682,6 +664,39 @@ int tasep_run_resume(tasep_ctx *c, double t0, double h, uint64_t steps, double t
     if (!c) return fail(TASEP_ERR_ARG, "NULL context");
     int rc = prepare_run(c, t0, h, thr, patience, resume);
     if (rc) return rc;
+    if (use_lds(c)) {
+        // the whole run in one workgroup, in launches of about 2^27
+        // state-updates and at most 65536 steps
+        const int nt = lds_threads(c);
+        LdsInstance &in = *c->lds_inst_host;
+        in.off = 0;
+        in.n = c->n;
+        in.tv = c->tv ? 1 : 0;
+        in.rate_off = 0;
+        in.tv_off = 0;
+        in.id = 0;
+        HIP_OK(hipMemcpyAsync(c->lds_inst, c->lds_inst_host, sizeof(LdsInstance), hipMemcpyHostToDevice,
+                              c->stream));
+        LdsArgs a;
+        a.inst = c->lds_inst;
+        a.x = xcur(c);
+        a.rates = c->rates;
+        a.tv_type = c->tv_type;
+        a.tv_amp = c->tv_par;
+        a.tv_period = c->tv_par ? c->tv_par + kRateStride : nullptr;
+        a.tv_offset = c->tv_par ? c->tv_par + 2 * kRateStride : nullptr;
+        a.tv_c = c->tv_par ? c->tv_par + 3 * kRateStride : nullptr;
+        a.st = c->st;
+        const bool tv = c->tv || (c->integ == TASEP_RK4 && c->force_staged);
+        uint64_t chunk = ((uint64_t)1 << 27) >> c->n;
+        if (chunk > 65536) chunk = 65536;
+        for (uint64_t j = 0; j < steps; j += chunk) {
+            a.steps = steps - j < chunk ? steps - j : chunk;
+            launch_lds_any(c->prec, c->integ, tv, a, 1, nt, lds_bytes(c->n, (int)c->esize, c->integ), c->stream);
+        }
+        HIP_OK(hipGetLastError());
+        return finish_run(c, info, true);
+    }
     const bool watch = c->st_host->patience >= 0;
     // steps per host check of the exit flag: about 2^26 state-updates
     uint64_t chunk = ((uint64_t)1 << 26) >> c->n;
@@ -708,6 +723,15 @@ int tasep_set_option(tasep_ctx *c, int key, int64_t value) {
     case TASEP_OPT_TILED: c->tiled = value != 0; break;
     case TASEP_OPT_XCD_ORDER: c->xcd_order = value != 0; break;
     case TASEP_OPT_RK4_STAGED: c->force_staged = value != 0; break;
+    case TASEP_OPT_LDS:
+        if (value < 0 || value > 2) return fail(TASEP_ERR_ARG, "LDS mode: 0, 1 or 2");
+        c->lds = (int)value;
+        break;
+    case TASEP_OPT_LDS_THREADS:
+        if (value != 0 && (value < 64 || value > 1024 || (value & (value - 1))))
+            return fail(TASEP_ERR_ARG, "LDS threads: 0 or a power of two in [64, 1024]");
+        c->lds_threads = (int)value;
+        break;
     case TASEP_OPT_TILE_BLOCKS:
         if (value != 0 && (value < 8 || value > kMaxBlocks || value % 8))
             return fail(TASEP_ERR_ARG, "tile blocks: 0 or a multiple of 8 in [8, 8192]");
@@ -753,28 +777,48 @@ int tasep_step_profile(tasep_ctx *c, double t0, double h, int reps, tasep_kernel
 
 int tasep_moments(tasep_ctx *c, double *y, double *cov, double *mul, double *mass) {
     if (!c) return fail(TASEP_ERR_ARG, "NULL context");
-    const int n = c->n;
+    return device_moments(c->stream, xcur(c), c->n, c->prec, &c->mom_part, &c->mom_out, y, cov, mul, mass);
+}
+
+int tasep_get_stream(tasep_ctx *c, void **stream) {
+    if (!c || !stream) return fail(TASEP_ERR_ARG, "NULL argument");
+    *stream = (void *)c->stream;
+    return TASEP_OK;
+}
+
+int tasep_get_x_device(tasep_ctx *c, void **ptr) {
+    if (!c || !ptr) return fail(TASEP_ERR_ARG, "NULL argument");
+    *ptr = xcur(c);
+    return TASEP_OK;
+}
+
+}  // extern "C"
+
+// Moments of a state vector x (2^n entries of the given precision) on the
+// device; *part and *out are allocated on first use and kept by the caller.
+int tasep_host::device_moments(hipStream_t stream, const void *x, int n, int prec, double **part, double **out,
+                               double *y, double *cov, double *mul, double *mass) {
+    const uint64_t dim = (uint64_t)1 << n;
     const int nq = n * (n + 1) / 2 + 1;
-    const uint64_t rows = c->dim >> kMomRowBits ? c->dim >> kMomRowBits : 1;
+    const uint64_t rows = dim >> kMomRowBits ? dim >> kMomRowBits : 1;
     const int nb = (int)(rows < (uint64_t)kMomMaxBlocks ? rows : kMomMaxBlocks);
-    if (!c->mom_part) {
-        HIP_OK(hipMalloc(&c->mom_part, (size_t)kMomMaxBlocks * kMomMaxQ * sizeof(double)));
-        HIP_OK(hipMalloc(&c->mom_out, kMomMaxQ * sizeof(double)));
+    if (!*part) {
+        HIP_OK(hipMalloc(part, (size_t)kMomMaxBlocks * kMomMaxQ * sizeof(double)));
+        HIP_OK(hipMalloc(out, kMomMaxQ * sizeof(double)));
     }
-    if (c->prec == TASEP_F32)
-        hipLaunchKernelGGL(k_moments_partial<float>, dim3(nb), dim3(kBlock), 0, c->stream,
-                           (const float *)xcur(c), n, c->dim, rows, c->mom_part);
+    if (prec == TASEP_F32)
+        hipLaunchKernelGGL(k_moments_partial<float>, dim3(nb), dim3(kBlock), 0, stream, (const float *)x, n, dim,
+                           rows, *part);
     else
-        hipLaunchKernelGGL(k_moments_partial<double>, dim3(nb), dim3(kBlock), 0, c->stream,
-                           (const double *)xcur(c), n, c->dim, rows, c->mom_part);
-    hipLaunchKernelGGL(k_moments_final, dim3(nq), dim3(kBlock), 0, c->stream, c->mom_part, nb, nq,
-                       c->mom_out);
+        hipLaunchKernelGGL(k_moments_partial<double>, dim3(nb), dim3(kBlock), 0, stream, (const double *)x, n, dim,
+                           rows, *part);
+    hipLaunchKernelGGL(k_moments_final, dim3(nq), dim3(kBlock), 0, stream, *part, nb, nq, *out);
     HIP_OK(hipGetLastError());
     std::vector<double> E(nq);
-    HIP_OK(hipMemcpyAsync(E.data(), c->mom_out, nq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
+    HIP_OK(hipMemcpyAsync(E.data(), *out, nq * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
     double M = E[nq - 1];
-    if (c->prec == TASEP_F32 && M > 0.0) {
+    if (prec == TASEP_F32 && M > 0.0) {
         for (int q = 0; q < nq - 1; q++) E[q] /= M;
         M = 1.0;
     }
@@ -793,17 +837,3 @@ int tasep_moments(tasep_ctx *c, double *y, double *cov, double *mul, double *mas
     }
     return TASEP_OK;
 }
-
-int tasep_get_stream(tasep_ctx *c, void **stream) {
-    if (!c || !stream) return fail(TASEP_ERR_ARG, "NULL argument");
-    *stream = (void *)c->stream;
-    return TASEP_OK;
-}
-
-int tasep_get_x_device(tasep_ctx *c, void **ptr) {
-    if (!c || !ptr) return fail(TASEP_ERR_ARG, "NULL argument");
-    *ptr = xcur(c);
-    return TASEP_OK;
-}
-
-}  // extern "C"

@@ -238,6 +238,7 @@ def test_converged_rk4_matches_dehp():
 
 # ----------------------------------------------------- tiled kernel coverage
 def _rk4_run(n, prec, r, steps, **opts):
+    opts.setdefault("lds", 0)   # these tests cover the streaming kernels
     with T.Engine(n, prec, "rk4", "mf") as e:
         for k, v in opts.items():
             e.set_option(getattr(T, "OPT_" + k.upper()), int(v))
@@ -301,6 +302,119 @@ def test_time_variant_tiled_vs_oracle():
         assert np.max(np.abs(x - ox)) <= tol, prec
 
 
+# ------------------------------------------- LDS-resident whole-run kernel
+def _engine_run(n, prec, integ, r=None, tv=None, x0=None, steps=100, h=0.01, thr=1e-7,
+                patience=-1, pieces=1, **opts):
+    with T.Engine(n, prec, integ, "mf") as e:
+        for k, v in opts.items():
+            e.set_option(getattr(T, "OPT_" + k.upper()), int(v))
+        if tv is not None:
+            e.set_time_variant(tv)
+        else:
+            e.set_rates(r)
+        if x0 is None:
+            e.set_x0_state(0)
+        else:
+            e.set_x(x0)
+        per = steps // pieces
+        for i in range(pieces):
+            k = per if i < pieces - 1 else steps - per * (pieces - 1)
+            info = e.run(0.0, h, k, thr=thr, patience=patience, resume=i > 0)
+            if info.stopped:
+                break
+        return e.get_x(), info
+
+
+def _tv(n, seed):
+    rng = np.random.default_rng(seed)
+    return dict(type=(np.arange(n + 1) % 3).astype(np.int32), amp=rng.uniform(0, 0.5, n + 1),
+                period=rng.uniform(0.5, 2, n + 1), offset=rng.uniform(0, 1, n + 1),
+                c=np.ones(n + 1))
+
+
+@pytest.mark.parametrize("n", [2, 3, 6, 9, 11, 13])
+def test_lds_ref_bit_exact_vs_oracle_and_streaming(n):
+    r = np.random.default_rng(200 + n).uniform(0.1, 2.0, n + 1)
+    x0 = np.random.default_rng(300 + n).random(1 << n)
+    x0 /= x0.sum()
+    ox, oinfo = oracle.run_ref(n, r, x0, 0.0, 0.01, 120, patience=-1)
+    xl, il = _engine_run(n, "f64", "ref", r, x0=x0, steps=120, lds=2)
+    xs, is_ = _engine_run(n, "f64", "ref", r, x0=x0, steps=120, lds=0)
+    assert _bits_equal(xl, ox), f"max|dx| = {np.max(np.abs(xl - ox))}"
+    assert _bits_equal(xs, ox)
+    assert il.t == oinfo.t and il.steps_done == 120
+    assert abs(il.last_checksum - oinfo.last_checksum) <= 1e-14 * oinfo.last_checksum
+    assert abs(il.mass - 1.0) <= 1e-12
+
+
+@pytest.mark.parametrize("threads", [64, 128, 512, 1024])
+def test_lds_workgroup_size_does_not_change_results(threads):
+    n = 12
+    r = np.random.default_rng(9).uniform(0.1, 2.0, n + 1)
+    base, _ = _engine_run(n, "f64", "ref", r, steps=80, lds=0)
+    x, _ = _engine_run(n, "f64", "ref", r, steps=80, lds=2, lds_threads=threads)
+    assert _bits_equal(x, base)
+
+
+def test_lds_ref_early_stop_and_chunks_match_streaming():
+    """Config-1 rates from the empty lattice: 10000 steps (early exit armed,
+    not reached) in one call and in 7 resumed pieces, LDS vs streaming."""
+    n = 10
+    r = np.array([0.5] + [1.0] * (n - 1) + [0.5])
+    h = oracle.step_h(100.0, 0.0, 10000)
+    a, ia = _engine_run(n, "f64", "ref", r, steps=10000, h=h, patience=100, lds=1)
+    b, ib = _engine_run(n, "f64", "ref", r, steps=10000, h=h, patience=100, lds=1, pieces=7)
+    c, ic = _engine_run(n, "f64", "ref", r, steps=10000, h=h, patience=100, lds=0)
+    assert _bits_equal(a, c) and _bits_equal(b, c)
+    assert ia.steps_done == ib.steps_done == ic.steps_done
+    assert ia.t == ib.t == ic.t
+    # early exit from the stationary uniform state: leaves at index 100
+    u, iu = _engine_run(n, "f64", "ref", r, x0=oracle.uniform_x0(n), steps=10000, h=h,
+                        patience=100, lds=1)
+    assert iu.stopped and iu.steps_at_stop == 100 and iu.steps_done == 101
+
+
+def test_lds_converged_exit_matches_streaming():
+    n = 8
+    r = np.random.default_rng(31).uniform(0.3, 1.5, n + 1)
+    a, ia = _engine_run(n, "f64", "ref", r, steps=100000, h=0.02, thr=1e-7, patience=100, lds=1)
+    b, ib = _engine_run(n, "f64", "ref", r, steps=100000, h=0.02, thr=1e-7, patience=100, lds=0)
+    assert ia.stopped and ib.stopped
+    assert ia.steps_at_stop == ib.steps_at_stop and ia.steps_done == ib.steps_done
+    assert _bits_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [4, 8, 12, 13])
+def test_lds_rk4_vs_oracle(n):
+    r = np.random.default_rng(400 + n).uniform(0.2, 1.5, n + 1)
+    x0 = np.zeros(1 << n)
+    x0[0] = 1.0
+    ox, _ = oracle.run_rk4(n, r, x0, 0.0, 0.01, 150)
+    x64, _ = _engine_run(n, "f64", "rk4", r, steps=150, lds=2)
+    x32, _ = _engine_run(n, "f32", "rk4", r, steps=150, lds=2)
+    assert np.max(np.abs(x64 - ox)) <= 1e-12
+    err = np.max(np.abs(x32 - ox))
+    print(f"n={n} FP32 LDS max|x - x_f64| = {err:.3e}")
+    assert err <= 1e-6
+    s64, _ = _engine_run(n, "f64", "rk4", r, steps=150, lds=2, rk4_staged=1)
+    assert np.max(np.abs(s64 - x64)) <= 1e-14
+
+
+@pytest.mark.parametrize("n", [7, 13])
+def test_lds_time_variant_vs_oracle(n):
+    tv = _tv(n, 50 + n)
+    x0 = oracle.uniform_x0(n)
+    ox, _ = oracle.run_rk4(n, None, x0, 0.0, 0.01, 120, tv=tv)
+    for prec, tol in (("f64", 1e-12), ("f32", 2e-7)):
+        x, info = _engine_run(n, prec, "rk4", tv=tv, x0=x0, steps=120, patience=100, lds=2)
+        assert info.steps_done == 120 and not info.stopped
+        assert np.max(np.abs(x - ox)) <= tol, prec
+    # reference loop with time-variant rates: LDS and streaming agree bitwise
+    a, _ = _engine_run(n, "f64", "ref", tv=tv, x0=x0, steps=120, lds=2)
+    b, _ = _engine_run(n, "f64", "ref", tv=tv, x0=x0, steps=120, lds=0)
+    assert _bits_equal(a, b)
+
+
 # ------------------------------------------------------------ error handling
 def test_errors_are_reported():
     with pytest.raises(T.TasepError):
@@ -314,3 +428,7 @@ def test_errors_are_reported():
             e.set_rates(np.array([0.5, -1, 1, 1, 1, 1, 0.5]))
         with pytest.raises(T.TasepError):
             e.set_x0_state(1 << 6)
+        with pytest.raises(T.TasepError):
+            e.set_option(T.OPT_LDS_THREADS, 96)   # not a power of two
+        with pytest.raises(T.TasepError):
+            e.set_option(T.OPT_LDS, 3)
