@@ -13,9 +13,9 @@
 // Constant-rate instances take RK4 in Horner form (passes x->u, u->v, v->u,
 // u->x), time-variant ones RK4 with stored stages (accumulator vector acc,
 // rates evaluated in FP64 per block at the pass's time).  The last pass
-// reduces sum|dx| and sum x per instance and advances each instance's run
-// state (t, early exit) in its last block, so a captured sequence of passes
-// replays without host work.
+// leaves per-block sums of |dx| and x; k_batch_finish reduces them per
+// instance and advances each run state (t, early exit) on the device, so a
+// captured sequence of steps replays without host work.
 #pragma once
 
 #include "device_common.hpp"
@@ -182,8 +182,27 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_pass(const BatchArgs<V>
     if (p == 3) {
         const double bc = block_sum<kBatchThreads>(ck.value(), red);
         const double bm = block_sum<kBatchThreads>(km.value(), red);
-        finish_group<kBatchThreads>(st, a.pchk + in.part_off, a.pmass + in.part_off, bb.blk, in.nblocks, bc, bm);
+        if (t == 0) {
+            a.pchk[in.part_off + bb.blk] = bc;
+            a.pmass[in.part_off + bb.blk] = bm;
+        }
     }
+}
+
+// After the last pass of a step: one workgroup per streamed instance sums
+// its block partials in a fixed order and advances its run state.  (A
+// separate launch instead of a last-block-done count: on gfx950 each
+// block's device-scope release would write back its XCD's L2.)
+__global__ __launch_bounds__(kBatchThreads) void k_batch_finish(const BatchInst *__restrict__ inst,
+                                                                RunState *__restrict__ st_all,
+                                                                const double *__restrict__ pchk,
+                                                                const double *__restrict__ pmass) {
+    const BatchInst in = inst[blockIdx.x];
+    RunState *st = st_all + in.id;
+    if (st->stopped) return;
+    double chk, mass;
+    reduce_partials<kBatchThreads>(pchk + in.part_off, pmass + in.part_off, in.nblocks, chk, mass);
+    if (threadIdx.x == 0) advance_state(st, chk, mass);
 }
 
 }  // namespace tasep

@@ -72,60 +72,53 @@ __device__ __forceinline__ double load_relaxed(const double *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// End-of-step bookkeeping, called by every block of a group of `nslots`
-// blocks (one instance's blocks of a step's final kernel) with its block
-// partials of sum|dx| and sum x, stored at slot `slot`.  The last block to
-// arrive reduces all partials in a fixed order (so the result does not
-// depend on which block is last) and advances the run state exactly as the
-// reference loop does (src/masterEquation.c:644,670-679): t += h, then the
-// sub-threshold counter and the exit test.
-template <int NT = kBlock>
-__device__ void finish_group(RunState *st, double *part_chk, double *part_mass, uint32_t slot, uint32_t nslots,
-                             double blk_chk, double blk_mass) {
-    __shared__ int s_last;
-    __shared__ double s_red[NT / kWave];
-    if (threadIdx.x == 0) {
-        part_chk[slot] = blk_chk;
-        part_mass[slot] = blk_mass;
-        __threadfence();
-        uint32_t prev = atomicAdd(&st->counter, 1u);
-        s_last = (prev == nslots - 1);
+// Advance a run state by one finished step with its checksum sum|dx| and
+// mass sum x, exactly as the reference loop does
+// (src/masterEquation.c:644,670-679): t += h, then the sub-threshold counter
+// and the exit test.  Called by one thread.
+__device__ __forceinline__ void advance_state(RunState *st, double chk, double mass) {
+    st->checksum = chk;
+    st->mass = mass;
+    st->t = st->t + st->h;
+    uint64_t i = st->steps_done;       // index of this step in the run
+    st->steps_done = i + 1;
+    if (st->patience >= 0) {
+        if (chk < st->thr_h) {
+            st->early_stop += 1;
+            st->steps_at_stop = i;
+        } else {
+            st->early_stop = 0;
+        }
+        if (st->early_stop > st->patience) st->stopped = 1;
     }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+    if (st->renorm && mass > 0.0) st->scale = 1.0 / mass;
+}
+
+// Fixed-order FP64 sums of nslots block partials (every thread of the
+// block gets both), so the result does not depend on block scheduling.
+template <int NT = kBlock>
+__device__ __forceinline__ void reduce_partials(const double *part_chk, const double *part_mass, uint32_t nslots,
+                                                double &chk, double &mass) {
+    __shared__ double s_red[NT / kWave];
     Kahan kc, km;
     for (uint32_t b = threadIdx.x; b < nslots; b += NT) {
         kc.add(load_relaxed(part_chk + b));
         km.add(load_relaxed(part_mass + b));
     }
-    double chk = block_sum<NT>(kc.value(), s_red);
-    double mass = block_sum<NT>(km.value(), s_red);
-    if (threadIdx.x == 0) {
-        st->checksum = chk;
-        st->mass = mass;
-        st->t = st->t + st->h;
-        uint64_t i = st->steps_done;       // index of this step in the run
-        st->steps_done = i + 1;
-        if (st->patience >= 0) {
-            if (chk < st->thr_h) {
-                st->early_stop += 1;
-                st->steps_at_stop = i;
-            } else {
-                st->early_stop = 0;
-            }
-            if (st->early_stop > st->patience) st->stopped = 1;
-        }
-        if (st->renorm && mass > 0.0) st->scale = 1.0 / mass;
-        st->counter = 0u;
-        __threadfence();
-    }
+    chk = block_sum<NT>(kc.value(), s_red);
+    mass = block_sum<NT>(km.value(), s_red);
 }
 
-// finish_group over the whole grid of the step's final kernel.
-template <int NT = kBlock>
-__device__ void finish_step(RunState *st, double *part_chk, double *part_mass, double blk_chk, double blk_mass) {
-    finish_group<NT>(st, part_chk, part_mass, blockIdx.x, gridDim.x, blk_chk, blk_mass);
+// Block partials of a step's final kernel: thread 0 stores the block's
+// sum|dx| and sum x at slot blockIdx.x; a separate launch (k_finish_step)
+// reduces them and advances the run state.  (No last-block-done count: on
+// gfx950 a device-scope release in every block writes back its XCD's L2.)
+__device__ __forceinline__ void store_partials(double *part_chk, double *part_mass, double blk_chk,
+                                               double blk_mass) {
+    if (threadIdx.x == 0) {
+        part_chk[blockIdx.x] = blk_chk;
+        part_mass[blockIdx.x] = blk_mass;
+    }
 }
 
 // Stage the rate set `set` (n+1 doubles) into LDS.

@@ -46,7 +46,17 @@ __global__ __launch_bounds__(kBlock) void k_ref_step(const RefArgs a) {
     }
     double bc = block_sum(chk.value(), red);
     double bm = block_sum(mass.value(), red);
-    finish_step(a.st, a.pchk, a.pmass, bc, bm);
+    store_partials(a.pchk, a.pmass, bc, bm);
+}
+
+// End of a step: sum the final kernel's `nblocks` block partials in a fixed
+// order and advance the run state (t += h, early-exit counter and test).
+__global__ __launch_bounds__(kBlock) void k_finish_step(RunState *st, const double *__restrict__ pchk,
+                                                        const double *__restrict__ pmass, uint32_t nblocks) {
+    if (st->stopped) return;
+    double chk, mass;
+    reduce_partials(pchk, pmass, nblocks, chk, mass);
+    if (threadIdx.x == 0) advance_state(st, chk, mass);
 }
 
 // ===========================================================================
@@ -149,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_rk_pass(const PassArgs<V> a) {
     if constexpr (EPI != EPI_STORE) {
         double bc = block_sum(chk.value(), red);
         double bm = block_sum(mass.value(), red);
-        finish_step(a.st, a.pchk, a.pmass, bc, bm);
+        store_partials(a.pchk, a.pmass, bc, bm);
     }
 }
 

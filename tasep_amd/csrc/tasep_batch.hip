@@ -186,7 +186,7 @@ BatchArgs<V> batch_args(tasep_batch *b, int pass) {
     return a;
 }
 
-// The streamed instances' passes of one step.
+// The streamed instances' passes of one step and their run-state update.
 void enqueue_stream_step(tasep_batch *b, hipStream_t s) {
     for (int p = 0; p < 4; p++) {
         if (b->prec == TASEP_F32)
@@ -196,6 +196,8 @@ void enqueue_stream_step(tasep_batch *b, hipStream_t s) {
             hipLaunchKernelGGL(k_batch_pass<double>, dim3(b->nblocks), dim3(kBatchThreads), 0, s,
                                batch_args<double>(b, p));
     }
+    hipLaunchKernelGGL(k_batch_finish, dim3((unsigned)b->str_ids.size()), dim3(kBatchThreads), 0, s, b->btab,
+                       b->st, b->pchk, b->pmass);
 }
 
 int build_graph(tasep_batch *b, int steps) {

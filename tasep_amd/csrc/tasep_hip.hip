@@ -363,12 +363,20 @@ static void prof_end(tasep_ctx *c, const char *name, double bytes_per_state) {
     }
 }
 
+// End-of-step reduction of the final kernel's nblocks partials.
+static void launch_finish(tasep_ctx *c, uint32_t nblocks) {
+    prof_begin(c);
+    hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(kBlock), 0, c->stream, c->st, c->pchk, c->pmass, nblocks);
+    prof_end(c, "finish_step", 0.0);
+}
+
 template <class V, bool CSR, bool TWO, int EPI, bool SELF = false>
 static void launch_pass(tasep_ctx *c, const PassArgs<V> &a, const char *name, double bytes) {
     prof_begin(c);
     hipLaunchKernelGGL((k_rk_pass<V, CSR, TWO, EPI, SELF>), dim3(grid_for(c->dim)), dim3(kBlock), 0,
                        c->stream, a);
     prof_end(c, name, bytes);
+    if (EPI != EPI_STORE) launch_finish(c, (uint32_t)grid_for(c->dim));
 }
 
 // One classical RK4 step in Horner form (constant rates; x updated in place).
@@ -465,6 +473,7 @@ static void launch_tile(tasep_ctx *c, TileArgs<V> a, const char *name, double by
     prof_begin(c);
     hipLaunchKernelGGL((k_tile_pass<V, TWO, EPI, SELF>), dim3(g), dim3(kTileThreads), 0, c->stream, a);
     prof_end(c, name, bytes);
+    if (EPI != EPI_STORE) launch_finish(c, (uint32_t)g);
 }
 
 template <class V>
@@ -534,6 +543,7 @@ static void ref_step(tasep_ctx *c, double h, uint64_t j) {
         hipLaunchKernelGGL(k_ref_step<false>, dim3(grid_for(c->dim)), dim3(kBlock), 0, c->stream, a);
         prof_end(c, "ref_step_mf", 16.0);
     }
+    launch_finish(c, (uint32_t)grid_for(c->dim));
 }
 
 static int lds_threads(const tasep_ctx *c) {

@@ -432,7 +432,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pass(const TileArgs<V> a)
     if constexpr (EPI != EPI_STORE) {
         const double bc = block_sum<kTileThreads>(chk.value(), red);
         const double bm = block_sum<kTileThreads>(mass.value(), red);
-        finish_step<kTileThreads>(a.st, a.pchk, a.pmass, bc, bm);
+        store_partials(a.pchk, a.pmass, bc, bm);
     }
 }
 
