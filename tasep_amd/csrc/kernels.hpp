@@ -49,14 +49,23 @@ __global__ __launch_bounds__(kBlock) void k_ref_step(const RefArgs a) {
     store_partials(a.pchk, a.pmass, bc, bm);
 }
 
-// End of a step: sum the final kernel's `nblocks` block partials in a fixed
-// order and advance the run state (t += h, early-exit counter and test).
+// End of `steps` steps: sum the last final kernel's `nblocks` block
+// partials in a fixed order and advance the run state (t += h per step,
+// early-exit counter and test).  steps > 1 only for runs without an exit
+// test, whose intermediate checksums are not needed.
 __global__ __launch_bounds__(kBlock) void k_finish_step(RunState *st, const double *__restrict__ pchk,
-                                                        const double *__restrict__ pmass, uint32_t nblocks) {
+                                                        const double *__restrict__ pmass, uint32_t nblocks,
+                                                        uint32_t steps) {
     if (st->stopped) return;
     double chk, mass;
     reduce_partials(pchk, pmass, nblocks, chk, mass);
-    if (threadIdx.x == 0) advance_state(st, chk, mass);
+    if (threadIdx.x == 0) {
+        for (uint32_t i = 1; i < steps; i++) {
+            st->t = st->t + st->h;
+            st->steps_done += 1;
+        }
+        advance_state(st, chk, mass);
+    }
 }
 
 // ===========================================================================

@@ -51,6 +51,9 @@ struct tasep_ctx {
     int lds = 1;                // whole runs in one workgroup's LDS: 0 never, 1 n <= 12, 2 where it fits
     int lds_threads = 0;        // workgroup size of the LDS-resident kernel (0: automatic)
     uint64_t steps_before = 0;  // cumulative step count when the current run call began
+    bool finish_each = true;    // reduce every step (exit test or time-variant rates need it)
+    uint32_t pending = 0;       // steps whose end-of-step reduction is deferred
+    uint32_t pending_blocks = 0;
     LdsInstance *lds_inst = nullptr;       // device: the instance table of one entry
     LdsInstance *lds_inst_host = nullptr;  // pinned
 };
@@ -363,11 +366,26 @@ static void prof_end(tasep_ctx *c, const char *name, double bytes_per_state) {
     }
 }
 
-// End-of-step reduction of the final kernel's nblocks partials.
+// End-of-step reduction of the final kernel's nblocks partials.  Runs
+// without an exit test and with constant rates defer it to the end of the
+// run (one launch then accounts for all pending steps).
 static void launch_finish(tasep_ctx *c, uint32_t nblocks) {
+    if (!c->finish_each && !g_prof) {
+        c->pending += 1;
+        c->pending_blocks = nblocks;
+        return;
+    }
     prof_begin(c);
-    hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(kBlock), 0, c->stream, c->st, c->pchk, c->pmass, nblocks);
+    hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(kBlock), 0, c->stream, c->st, c->pchk, c->pmass, nblocks,
+                       1u);
     prof_end(c, "finish_step", 0.0);
+}
+
+static void flush_finish(tasep_ctx *c) {
+    if (!c->pending) return;
+    hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(kBlock), 0, c->stream, c->st, c->pchk, c->pmass,
+                       c->pending_blocks, c->pending);
+    c->pending = 0;
 }
 
 template <class V, bool CSR, bool TWO, int EPI, bool SELF = false>
@@ -564,6 +582,8 @@ static bool use_lds(const tasep_ctx *c) {
 static int prepare_run(tasep_ctx *c, double t0, double h, double thr, int patience, int resume = 0) {
     if (!c->have_rates) return fail(TASEP_ERR_STATE, "set rates before running");
     if (!isfinite(h) || !isfinite(t0)) return fail(TASEP_ERR_ARG, "t0 and h must be finite");
+    c->pending = 0;
+    c->finish_each = c->tv || patience >= 0;
     if (c->gen == TASEP_CSR && !c->csr_built) {
         int rc = tasep_build_csr(c);
         if (rc) return rc;
@@ -643,6 +663,7 @@ static void enqueue_step(tasep_ctx *c, double h, uint64_t j) {
 
 // Read the loop state back and move the ping-pong index.
 static int finish_run(tasep_ctx *c, tasep_run_info *info, bool inplace = false) {
+    flush_finish(c);
     HIP_OK(hipMemcpyAsync(c->st_host, c->st, sizeof(RunState), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     const RunState &s = *c->st_host;
