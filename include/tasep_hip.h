@@ -134,6 +134,15 @@ int tasep_run_resume(tasep_ctx *ctx, double t0, double h, uint64_t steps,
 int tasep_moments(tasep_ctx *ctx, double *y, double *cov, double *mul,
                   double *mass);
 
+/* Window marginals of the current state in one device pass with FP64 sums:
+ *   W[o*16 + p] = sum_s x_s [((s >> o) & 15) == p],  o = 0..n-4, p = 0..15
+ * ((n-3)*16 doubles; n >= 4; FP32 contexts divided by the FP64 mass).
+ * Every high-order output of the reference -- key_fit/high_order_corr,
+ * error guess, closures, point functions (src/masterEquation.c:32-143,
+ * :1422-1539), each a loop over all 2^n states per printed value -- is a
+ * sum of these pattern probabilities over windows of <= 4 sites. */
+int tasep_window_marginals(tasep_ctx *ctx, double *W);
+
 /* Per-launch timing of one step, for measurement: runs `reps` steps from t0
  * (advancing the state like tasep_run with patience < 0), timing every
  * kernel launch of a step with HIP events on the context's stream, and
@@ -211,9 +220,11 @@ int tasep_batch_get_x(tasep_batch *batch, int i, double *x);
  * `count` entries.  thr/patience as tasep_run, per instance. */
 int tasep_batch_run(tasep_batch *batch, double t0, double h, uint64_t steps,
                     double thr, int patience, tasep_run_info *info);
-/* Moments of instance i, as tasep_moments. */
+/* Moments and window marginals of instance i, as tasep_moments and
+ * tasep_window_marginals. */
 int tasep_batch_moments(tasep_batch *batch, int i, double *y, double *cov,
                         double *mul, double *mass);
+int tasep_batch_window_marginals(tasep_batch *batch, int i, double *W);
 /* Batch options (defaults in brackets):
  *   TASEP_BATCH_OPT_GRAPH_STEPS [32] steps per captured HIP graph of the
  *                                    shared passes; 0 = direct launches

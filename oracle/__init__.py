@@ -85,6 +85,9 @@ def lib():
         L.tor_cov.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p]
         L.tor_mass.restype = C.c_double
         L.tor_mass.argtypes = [C.c_uint64, _f64p]
+        L.tor_key_sum.restype = C.c_double
+        L.tor_key_sum.argtypes = [C.c_int, _f64p, C.c_uint64, C.c_int, C.c_int]
+        L.tor_windows.argtypes = [C.c_int, _f64p, _f64p]
         L.tor_dense.argtypes = [C.c_int, _f64p, _f64p]
         L.tor_ref_update4.restype = C.c_double
         L.tor_ref_update4.argtypes = [C.c_uint64, _f64p, _f64p, _f64p, _f64p, _f64p]
@@ -219,6 +222,18 @@ def cov(n, x, y=None):
     m = np.zeros(n * n, np.float64)
     lib().tor_cov(n, _f64(x), _f64(y), c, m)
     return c.reshape(n, n), m.reshape(n, n)
+
+
+def key_sum(n, x, key, off, size):
+    """The reference's key_fit sum over all states (masterEquation.c:32-45)."""
+    return float(lib().tor_key_sum(n, _f64(x), key, off, size))
+
+
+def windows(n, x):
+    """W[o, p] = sum_s x_s [((s >> o) & 15) == p], o = 0..n-4."""
+    W = np.zeros((n - 3) * 16, np.float64)
+    lib().tor_windows(n, _f64(x), W)
+    return W.reshape(n - 3, 16)
 
 
 def mass(x):

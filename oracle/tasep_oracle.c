@@ -325,6 +325,33 @@ void tor_cov(int n, const double *x, const double *y, double *cov, double *mul)
     }
 }
 
+/* The reference's key_fit sum (masterEquation.c:32-45, summed over all
+ * states as in the loops of :61-68, :125-128): sum of x_s over the states
+ * whose bits off..off+size-1 equal the low `size` bits of key. */
+double tor_key_sum(int n, const double *x, uint64_t key, int off, int size)
+{
+    uint64_t dim = (uint64_t)1 << n;
+    double acc = 0.;
+    for (uint64_t s = 0; s < dim; s++) {
+        int fit = 1;
+        for (int i = 0; i < size; i++)
+            if (((key >> i) & 1) != ((s >> (off + i)) & 1)) fit = 0;
+        acc += fit ? x[s] : 0.;
+    }
+    return acc;
+}
+
+/* All 4-site window marginals: W[o*16 + p] = sum_s x_s [((s >> o) & 15) == p]
+ * for o = 0..n-4 (no reference counterpart: the reference recomputes each
+ * key_fit sum over all states; each of its sums is a sum of W entries). */
+void tor_windows(int n, const double *x, double *W)
+{
+    uint64_t dim = (uint64_t)1 << n;
+    for (int q = 0; q < (n - 3) * 16; q++) W[q] = 0.;
+    for (uint64_t s = 0; s < dim; s++)
+        for (int o = 0; o + 4 <= n; o++) W[o * 16 + (int)((s >> o) & 15)] += x[s];
+}
+
 double tor_mass(uint64_t dim, const double *x)
 {
     double m = 0;

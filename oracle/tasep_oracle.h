@@ -104,6 +104,8 @@ void tor_site_moments(int n, const double *x, double *y);
 void tor_cov(int n, const double *x, const double *y, double *cov, double *mul);
 
 /* Sequential sum of x (src/masterEquation.c:1009-1011). */
+double tor_key_sum(int n, const double *x, uint64_t key, int off, int size);
+void tor_windows(int n, const double *x, double *W);
 double tor_mass(uint64_t dim, const double *x);
 
 /* The reference's update with four stage arrays, as written at

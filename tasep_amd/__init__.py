@@ -34,7 +34,8 @@ EXPORTS = [
     "tasep_batch_create", "tasep_batch_destroy", "tasep_batch_set_rates",
     "tasep_batch_set_time_variant", "tasep_batch_set_x0_uniform", "tasep_batch_set_x",
     "tasep_batch_get_x", "tasep_batch_run", "tasep_batch_moments",
-    "tasep_batch_set_option", "tasep_batch_get_stream",
+    "tasep_batch_set_option", "tasep_batch_get_stream", "tasep_window_marginals",
+    "tasep_batch_window_marginals",
 ]
 OPT_TILED, OPT_XCD_ORDER, OPT_RK4_STAGED, OPT_TILE_BLOCKS, OPT_LDS, OPT_LDS_THREADS = 1, 2, 3, 4, 5, 6
 BATCH_OPT_GRAPH_STEPS, BATCH_OPT_LDS, BATCH_OPT_CONCURRENT = 1, 2, 3
@@ -123,6 +124,8 @@ def load_library():
     L.tasep_batch_moments.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     L.tasep_batch_set_option.argtypes = [vp, C.c_int, C.c_int64]
     L.tasep_batch_get_stream.argtypes = [vp, C.POINTER(vp)]
+    L.tasep_window_marginals.argtypes = [vp, _f64p]
+    L.tasep_batch_window_marginals.argtypes = [vp, C.c_int, _f64p]
     for name in EXPORTS:
         if name not in ("tasep_strerror", "tasep_csr_nnz"):
             getattr(L, name).restype = C.c_int
@@ -249,6 +252,12 @@ class Engine:
                                             mul.ctypes.data, m.ctypes.data))
         return y, cov.reshape(n, n), mul.reshape(n, n), float(m[0])
 
+    def window_marginals(self):
+        """W[o, p] = sum_s x_s [((s >> o) & 15) == p], o = 0..n-4."""
+        W = np.zeros((self.n - 3) * 16)
+        _check(load_library().tasep_window_marginals(self._h, W))
+        return W.reshape(self.n - 3, 16)
+
     def step_profile(self, t0, h, reps=5):
         """[(name, mean ms, algorithmic bytes/state)] for each launch of a step."""
         arr = (_KernelTime * 8)()
@@ -350,6 +359,11 @@ class Batch:
         _check(load_library().tasep_batch_moments(self._h, i, y.ctypes.data, cov.ctypes.data,
                                                   mul.ctypes.data, m.ctypes.data))
         return y, cov.reshape(n, n), mul.reshape(n, n), float(m[0])
+
+    def window_marginals(self, i):
+        W = np.zeros(max(self.ns[i] - 3, 0) * 16)
+        _check(load_library().tasep_batch_window_marginals(self._h, i, W))
+        return W.reshape(-1, 16)
 
     @property
     def stream(self) -> int:

@@ -60,4 +60,8 @@ inline uint64_t grid_for(uint64_t dim, int cap = kMaxBlocks) {
 int device_moments(hipStream_t stream, const void *x, int n, int prec, double **part, double **out, double *y,
                    double *cov, double *mul, double *mass);
 
+// Window marginals W[o*16 + p] (o = 0..n-4) of a device state vector, as
+// tasep_window_marginals.
+int device_windows(hipStream_t stream, const void *x, int n, int prec, double **part, double **out, double *W);
+
 }  // namespace tasep_host

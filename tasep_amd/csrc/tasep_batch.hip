@@ -410,6 +410,15 @@ int tasep_batch_moments(tasep_batch *b, int i, double *y, double *cov, double *m
     return device_moments(b->stream, x, b->n[i], b->prec, &b->mom_part, &b->mom_out, y, cov, mul, mass);
 }
 
+int tasep_batch_window_marginals(tasep_batch *b, int i, double *W) {
+    int rc = check_index(b, i);
+    if (rc) return rc;
+    if (!W) return fail(TASEP_ERR_ARG, "NULL argument");
+    if (b->n[i] < 4) return fail(TASEP_ERR_ARG, "window marginals need n >= 4");
+    const void *x = (const char *)b->x + b->off[i] * b->esize;
+    return device_windows(b->stream, x, b->n[i], b->prec, &b->mom_part, &b->mom_out, W);
+}
+
 int tasep_batch_set_option(tasep_batch *b, int key, int64_t value) {
     if (!b) return fail(TASEP_ERR_ARG, "NULL batch");
     switch (key) {

@@ -868,3 +868,37 @@ int tasep_host::device_moments(hipStream_t stream, const void *x, int n, int pre
     }
     return TASEP_OK;
 }
+
+// Window marginals of a state vector (2^n entries, n >= 4) on the device:
+// W[o*16 + p] for o = 0..n-4; FP32 vectors are divided by their FP64 mass.
+int tasep_host::device_windows(hipStream_t stream, const void *x, int n, int prec, double **part, double **out,
+                               double *W) {
+    const uint64_t dim = (uint64_t)1 << n;
+    const int nq = (n - kWin + 1) * kWinBins + 1;
+    const uint64_t rows = dim >> kMomRowBits ? dim >> kMomRowBits : 1;
+    const int nb = (int)(rows < (uint64_t)kMomMaxBlocks ? rows : kMomMaxBlocks);
+    if (!*part) {
+        HIP_OK(hipMalloc(part, (size_t)kMomMaxBlocks * kMomMaxQ * sizeof(double)));
+        HIP_OK(hipMalloc(out, kMomMaxQ * sizeof(double)));
+    }
+    if (prec == TASEP_F32)
+        hipLaunchKernelGGL(k_windows_partial<float>, dim3(nb), dim3(kBlock), 0, stream, (const float *)x, n, dim,
+                           rows, *part);
+    else
+        hipLaunchKernelGGL(k_windows_partial<double>, dim3(nb), dim3(kBlock), 0, stream, (const double *)x, n,
+                           dim, rows, *part);
+    hipLaunchKernelGGL(k_moments_final, dim3(nq), dim3(kBlock), 0, stream, *part, nb, nq, *out);
+    HIP_OK(hipGetLastError());
+    std::vector<double> E(nq);
+    HIP_OK(hipMemcpyAsync(E.data(), *out, nq * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    const double M = E[nq - 1];
+    for (int q = 0; q < nq - 1; q++) W[q] = (prec == TASEP_F32 && M > 0.0) ? E[q] / M : E[q];
+    return TASEP_OK;
+}
+
+extern "C" int tasep_window_marginals(tasep_ctx *c, double *W) {
+    if (!c || !W) return fail(TASEP_ERR_ARG, "NULL argument");
+    if (c->n < kWin) return fail(TASEP_ERR_ARG, "window marginals need n >= 4");
+    return device_windows(c->stream, xcur(c), c->n, c->prec, &c->mom_part, &c->mom_out, W);
+}

@@ -352,9 +352,20 @@ int main(int argc, char *argv[])
     gpu_check(tasep_moments(ctx, y, cov, mul, &mass), "computing moments");
     printf("Checksum over result vector: %lf\n", mass);
 
-    double *x = (double *)malloc(dim * sizeof(double));
-    if (!x) die("ERROR: Not enough memory to allocate x array", -1);
-    gpu_check(tasep_get_x(ctx, x), "reading the state vector");
+    /* the high-order outputs come from 4-site window marginals computed on
+     * the device; the state itself is copied only for the probabilities file */
+    double *W = NULL;
+    if (n >= 4) {
+        W = (double *)malloc((size_t)(n - 3) * 16 * sizeof(double));
+        if (!W) die("ERROR: Not enough memory", -1);
+        gpu_check(tasep_window_marginals(ctx, W), "computing window marginals");
+    }
+    double *x = NULL;
+    if (save_p) {
+        x = (double *)malloc(dim * sizeof(double));
+        if (!x) die("ERROR: Not enough memory to allocate x array", -1);
+        gpu_check(tasep_get_x(ctx, x), "reading the state vector");
+    }
 
     out_ctx_t oc;
     oc.n = n;
@@ -367,11 +378,11 @@ int main(int argc, char *argv[])
     oc.save_c = save_c;
     oc.save_m = save_m;
     oc.have_a = have_a;
-    write_outputs(&oc, x, y, cov, mul, &snaps);
+    write_outputs(&oc, x, y, cov, mul, W, &snaps);
 
     snapshots_free(&snaps);
     tasep_ctx_destroy(ctx);
-    free(x); free(y); free(cov); free(mul); free(rates);
+    free(x); free(W); free(y); free(cov); free(mul); free(rates);
     free(i_file); free(o_file); free(save_sel); free(tv_type);
     for (int p = 0; p < 5; p++) free(tv_file[p]);
     for (int p = 0; p < 4; p++) free(tv_par[p]);

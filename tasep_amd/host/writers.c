@@ -9,8 +9,10 @@
  * both axes reversed (entry site first).
  *
  * The post-processing at the end (error guess, closures, point functions,
- * the high-order correlation) runs here on the host over the final state,
- * as the reference does (src/masterEquation.c:32-143, 1422-1539).
+ * the high-order correlation, src/masterEquation.c:32-143, 1422-1539) sums
+ * 4-site window marginals computed on the device instead of looping over
+ * all 2^n states per value; the timed correlation of -a snapshots (6-site
+ * keys) still loops over the snapshot states.
  */
 #include "writers.h"
 
@@ -19,6 +21,8 @@
 #include <stdlib.h>
 #include <string.h>
 #include <sys/stat.h>
+
+#include <omp.h>
 
 /* ------------------------------------------------------------ snapshots */
 void snapshots_init(snapshots_t *s, int n, uint64_t dim, int res, unsigned int steps, int keep_x)
@@ -214,28 +218,97 @@ static double high_order_corr(const double *x, uint64_t dim, uint64_t key, int k
     return ret / sqrt(k.left * (k.no - k.left) * k.right * (k.no - k.right));
 }
 
-static double error_guess(const double *x, uint64_t dim, uint64_t key, int ks, int off)
+/* key_fit sum over states whose bits off..off+size-1 equal key (size <= 4),
+ * from the window marginals W[o*16 + p] = P(bits o..o+3 = p) */
+static double win_sum(const double *W, int n, uint64_t key, int off, int size)
 {
-    const keysums_t k = key_sums(x, dim, key, ks, off, 1);
+    const int w = off < n - 4 ? off : n - 4;
+    const int sh = off - w;
+    const unsigned m = (1u << size) - 1;
+    double acc = 0;
+    for (unsigned p = 0; p < 16; p++)
+        if (((p >> sh) & m) == (unsigned)key) acc += W[w * 16 + (int)p];
+    return acc;
+}
+
+/* the four pattern marginals of src/masterEquation.c:61-68 from W */
+static keysums_t win_sums(const double *W, int n, uint64_t key, int ks, int off)
+{
+    keysums_t r;
+    r.lr = win_sum(W, n, (key << 1) + 1 + ((uint64_t)1 << (ks + 1)), off - 1, ks + 2);
+    r.no = win_sum(W, n, key, off, ks);
+    r.left = win_sum(W, n, key + ((uint64_t)1 << ks), off, ks + 1);
+    r.right = win_sum(W, n, (key << 1) + 1, off - 1, ks + 1);
+    return r;
+}
+
+static double win_corr(const double *W, int n, uint64_t key, int ks, int off)
+{
+    const keysums_t k = win_sums(W, n, key, ks, off);
+    const double ret = k.lr * k.no - k.left * k.right;
+    return ret / sqrt(k.left * (k.no - k.left) * k.right * (k.no - k.right));
+}
+
+static double error_guess(const double *W, int n, uint64_t key, int ks, int off)
+{
+    const keysums_t k = win_sums(W, n, key, ks, off);
     return (k.lr / k.no - (k.left / k.no) * (k.right / k.no)) * k.no;
 }
 
-static double closure_value(const double *x, uint64_t dim, uint64_t key, int ks, int off)
+static double closure_value(const double *W, int n, uint64_t key, int ks, int off)
 {
-    const keysums_t k = key_sums(x, dim, key, ks, off, 0);
+    const keysums_t k = win_sums(W, n, key, ks, off);
     return (k.left * k.right) / k.no;
 }
 
-static double point_func(const double *x, uint64_t dim, uint64_t key, int ks, int off)
+static double point_func(const double *W, int n, uint64_t key, int ks, int off)
 {
-    double v = 0;
-    for (uint64_t s = 0; s < dim; s++) v += key_match(s, key, off, ks) ? x[s] : 0.;
-    return v;
+    return win_sum(W, n, key, off, ks);
 }
 
 /* ------------------------------------------------------------- writers */
+/* The probabilities file (src/masterEquation.c:1021-1040): one line
+ * "<bits, entry site first>\t\t%.20g" per state.  Lines are formatted in
+ * parallel, a block of lines per thread, and written in order, so the file
+ * is the same byte for byte (about 13 GB at n = 28). */
+static void write_probabilities(FILE *f, int n, uint64_t dim, const double *x)
+{
+    enum { PER = 1 << 16 };                       /* lines per thread and round */
+    const int nt = omp_get_max_threads() > 0 ? omp_get_max_threads() : 1;
+    const size_t cap = (size_t)PER * (size_t)(n + 32);  /* n bits + 2 tabs + %.20g + \n */
+    char **buf = (char **)malloc((size_t)nt * sizeof(char *));
+    size_t *len = (size_t *)calloc((size_t)nt, sizeof(size_t));
+    if (!buf || !len) {
+        printf("ERROR: Not enough memory\n");
+        exit(-1);
+    }
+    for (int t = 0; t < nt; t++)
+        if (!(buf[t] = (char *)malloc(cap))) {
+            printf("ERROR: Not enough memory\n");
+            exit(-1);
+        }
+    for (uint64_t base = 0; base < dim; base += (uint64_t)nt * PER) {
+#pragma omp parallel num_threads(nt)
+        {
+            const int t = omp_get_thread_num();
+            char *p = buf[t];
+            const uint64_t lo = base + (uint64_t)t * PER;
+            const uint64_t hi = lo + PER < dim ? lo + PER : dim;
+            for (uint64_t s = lo; s < hi; s++) {
+                for (int j = 0; j < n; j++) *p++ = ((s >> (n - j - 1)) & 1) ? '1' : '0';
+                p += sprintf(p, "\t\t%.20g\n", x[s]);
+            }
+            len[t] = lo < hi ? (size_t)(p - buf[t]) : 0;
+        }
+        for (int t = 0; t < nt; t++) fwrite(buf[t], 1, len[t], f);
+    }
+    for (int t = 0; t < nt; t++) free(buf[t]);
+    free(buf);
+    free(len);
+}
+
 void write_outputs(const out_ctx_t *oc, const double *x, const double *y, const double *cov,
-                   const double *mul, const snapshots_t *snaps)
+                   const double *mul, const double *win, const snapshots_t *snaps)
 {
     const int n = oc->n;
     const uint64_t dim = oc->dim;
@@ -250,14 +323,7 @@ void write_outputs(const out_ctx_t *oc, const double *x, const double *y, const 
         f = fopen(o, "w");
         if (f) {
             printf("Writing result vector to file '%s'\n", o);
-            static char buf[1 << 20];
-            setvbuf(f, buf, _IOFBF, sizeof buf);
-            char bits[72];
-            for (uint64_t s = 0; s < dim; s++) {
-                for (int j = 0; j < n; j++) bits[j] = ((s >> (n - j - 1)) & 1) ? '1' : '0';
-                bits[n] = 0;
-                fprintf(f, "%s\t\t%.20g\n", bits, x[s]);
-            }
+            write_probabilities(f, n, dim, x);
             printf("\n");
             fclose(f);
         }
@@ -361,7 +427,7 @@ void write_outputs(const out_ctx_t *oc, const double *x, const double *y, const 
                 for (int i = 0; i < ms; i++) fprintf(f, ((mask >> (ms - 1 - i)) & 1) ? "1" : "0");
                 fprintf(f, "\t");
                 for (int i = 1; i < n - ms; i++)
-                    fprintf(f, "%1.4g\t", high_order_corr(x, dim, mask, ms, n - ms - i));
+                    fprintf(f, "%1.4g\t", win_corr(win, n, mask, ms, n - ms - i));
                 fprintf(f, "\n");
             }
             fclose(f);
@@ -376,7 +442,7 @@ void write_outputs(const out_ctx_t *oc, const double *x, const double *y, const 
         if ((f = fopen(name, "w"))) {
             for (int i = 1; i < n - ms; i++)
                 fprintf(f, "Error guess for t_%d...t_%d:\t %lf\n", i - 1, i + ms,
-                        error_guess(x, dim, mask, ms, n - ms - i));
+                        error_guess(win, n, mask, ms, n - ms - i));
             fclose(f);
         }
     }
@@ -388,7 +454,7 @@ void write_outputs(const out_ctx_t *oc, const double *x, const double *y, const 
         if ((f = fopen(name, "w"))) {
             for (int i = 1; i < n - ms; i++)
                 fprintf(f, "Value for closure of t_%d...t_%d:\t %lf\n", i - 1, i + ms,
-                        closure_value(x, dim, mask, ms, n - ms - i));
+                        closure_value(win, n, mask, ms, n - ms - i));
             fclose(f);
         }
     }
@@ -400,7 +466,7 @@ void write_outputs(const out_ctx_t *oc, const double *x, const double *y, const 
         snprintf(name, (size_t)size, "%s_high_order_point_func", o);
         if ((f = fopen(name, "w"))) {
             for (int i = 0; i < n - ms + 1; i++)
-                fprintf(f, "t_%d...t_%d = %lf\n", i, ms + i - 1, point_func(x, dim, mask, ms, n - ms - i));
+                fprintf(f, "t_%d...t_%d = %lf\n", i, ms + i - 1, point_func(win, n, mask, ms, n - ms - i));
             fclose(f);
         }
     }

@@ -33,9 +33,11 @@ typedef struct {
     int save_e, save_p, save_c, save_m, have_a;
 } out_ctx_t;
 
-/* Writes every output file of the reference for the final state x with
- * moments y (by bit), cov and mul (by bit, [i*n + k]). */
+/* Writes every output file of the reference for the final state: x (read
+ * only for the probabilities file, may be NULL when it is not selected),
+ * moments y (by bit), cov and mul (by bit, [i*n + k]) and the 4-site window
+ * marginals win[o*16 + p] of tasep_window_marginals (NULL for n < 4). */
 void write_outputs(const out_ctx_t *oc, const double *x, const double *y, const double *cov,
-                   const double *mul, const snapshots_t *snaps);
+                   const double *mul, const double *win, const snapshots_t *snaps);
 
 #endif

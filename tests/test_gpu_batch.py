@@ -145,6 +145,7 @@ def test_batch_moments_vs_oracle():
             assert np.max(np.abs(y - oy)) <= 1e-13
             assert np.max(np.abs(cov[::-1, ::-1] - ocov)) <= 1e-13
             assert abs(mass - oracle.mass(x)) <= 1e-13
+            assert np.max(np.abs(b.window_marginals(i) - oracle.windows(n, x))) <= 1e-13
 
 
 def test_batch_errors():

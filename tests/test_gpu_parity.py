@@ -146,6 +146,21 @@ def test_moments_vs_oracle(n):
     assert np.max(np.abs(mul[::-1, ::-1] - omul)) <= 1e-13
 
 
+@pytest.mark.parametrize("n", [4, 5, 9, 10, 11, 13, 14, 17, 20])
+def test_window_marginals_vs_oracle(n):
+    x = np.random.default_rng(90 + n).random(1 << n)
+    x /= x.sum()
+    with T.Engine(n, "f64", "ref", "mf") as e:
+        e.set_x(x)
+        W = e.window_marginals()
+    assert np.max(np.abs(W - oracle.windows(n, x))) <= 1e-13
+    # FP32 contexts: divided by the FP64 mass of the stored vector
+    with T.Engine(n, "f32", "rk4", "mf") as e:
+        e.set_x(2.0 * x)
+        W32 = e.window_marginals()
+    assert np.max(np.abs(W32 - oracle.windows(n, x))) <= 1e-7
+
+
 # ------------------------------------------------------------------- RK4 mode
 @pytest.mark.parametrize("gen", ["mf", "csr"])
 def test_rk4_f64_vs_oracle(gen):
@@ -432,3 +447,6 @@ def test_errors_are_reported():
             e.set_option(T.OPT_LDS_THREADS, 96)   # not a power of two
         with pytest.raises(T.TasepError):
             e.set_option(T.OPT_LDS, 3)
+    with T.Engine(3, "f64", "ref", "mf") as e:
+        with pytest.raises(T.TasepError):
+            e.window_marginals()          # needs n >= 4

@@ -246,3 +246,48 @@ def test_gamma_rescaled_twin():
     r2 = [0.6] + [2.0] * (n - 1) + [0.8]
     p = dehp.steady_state_bruteforce(n, r2)
     assert dehp.densities_from_p(n, p) == dehp.densities(n, 0.6, 0.8, 2.0)
+
+
+# ------------------------------------------------- high-order post-processing
+def test_windows_equal_reference_key_sums():
+    """Every key_fit sum the reference's high-order functions take (<= 4
+    sites) is a sum of window marginals (oracle.key_sum restates key_fit)."""
+    from tests import _highorder as H
+    for n in (4, 5, 8):
+        x = np.random.default_rng(60 + n).random(1 << n)
+        W = oracle.windows(n, x)
+        for size in (1, 2, 3, 4):
+            for off in range(0, n - size + 1):
+                for key in range(1 << size):
+                    want = oracle.key_sum(n, x, key, off, size)
+                    assert abs(H.key_sum(W, n, key, off, size) - want) <= 1e-12 * x.sum()
+
+
+def test_windows_invariants():
+    n = 9
+    x = np.random.default_rng(3).random(1 << n)
+    W = oracle.windows(n, x)
+    assert np.allclose(W.sum(axis=1), x.sum(), rtol=1e-14, atol=0)
+    # the 3-site window o+1..o+3 from W[o] (drop bit 0) and W[o+1] (drop bit 3)
+    for o in range(n - 4):
+        a = W[o].reshape(8, 2).sum(axis=1)
+        b = W[o + 1].reshape(2, 8).sum(axis=0)
+        assert np.allclose(a, b, rtol=1e-13, atol=0)
+    # neighbour pair moments E[b_o b_{o+1}] by brute force
+    s = np.arange(1 << n)
+    for o in range(n - 3):
+        e = x[((s >> o) & 3) == 3].sum()
+        assert abs(W[o][[p for p in range(16) if p & 3 == 3]].sum() - e) <= 1e-13 * x.sum()
+
+
+@pytest.mark.parametrize("name", G.cli_case_names())
+def test_high_order_files_from_windows_match_reference(name):
+    """Error guess, closures and point functions of the reference's own runs,
+    recomputed from the window marginals of its final state, text-equal."""
+    from tests import _highorder as H
+    c = G.cli_case(name)
+    if c["n"] < 4:
+        pytest.skip("no windows below n = 4")
+    W = oracle.windows(c["n"], c["x"])
+    for fname, text in H.files(W, c["n"]).items():
+        assert text == G.read_text(c, fname), fname
