@@ -459,11 +459,10 @@ static TileArgs<V> tile_base(tasep_ctx *c) {
     return a;
 }
 
-// Grid of a tiled pass.  Store passes: 16 workgroups per resident slot
-// (occupancy x CUs), which balances the tail; the final pass of a step: one
-// resident wave of workgroups, so that its end-of-step reduction sees few
-// partials (measured at n=28: 0.92 vs 1.03 ms for the store passes, 1.00 vs
-// 1.14 ms for the final pass).  Rounded down to a multiple of 8 XCDs.
+// Grid of a tiled pass: 16 workgroups per resident slot (occupancy x
+// CUs), which balances the tail (measured at n=28: 0.92 vs 1.03 ms for one
+// resident wave), at most kMaxBlocks partials for the end-of-step
+// reduction.  Rounded down to a multiple of 8 XCDs.
 template <class V, bool TWO, int EPI, bool SELF>
 static int tile_grid_cap() {
     static int cap = 0;
@@ -475,7 +474,7 @@ static int tile_grid_cap() {
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             per_cu < 1 || cus < 1)
             return 1024;
-        cap = (per_cu * cus * (EPI == EPI_STORE ? 16 : 1)) & ~7;
+        cap = (per_cu * cus * 16) & ~7;
         if (cap < 8) cap = 8;
         if (cap > kMaxBlocks) cap = kMaxBlocks;
     }
@@ -539,7 +538,47 @@ static bool use_tile(const tasep_ctx *c) {
     return c->tiled && c->gen == TASEP_MF && c->n >= TileGeom<V>::b + 1;
 }
 
+static int tile_ref_cap() {
+    static int cap = 0;
+    if (!cap) {
+        int per_cu = 0, dev = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_ref, kTileThreads, 0) != hipSuccess ||
+            hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            per_cu < 1 || cus < 1)
+            return 1024;
+        cap = (per_cu * cus * 16) & ~7;
+        if (cap < 8) cap = 8;
+        if (cap > kMaxBlocks) cap = kMaxBlocks;
+    }
+    return cap;
+}
+
+static bool use_tile_ref(const tasep_ctx *c) {
+    return c->tiled && c->gen == TASEP_MF && c->n >= TileGeom<double>::b + 1;
+}
+
 static void ref_step(tasep_ctx *c, double h, uint64_t j) {
+    if (use_tile_ref(c)) {
+        TileRefArgs t;
+        t.x = (const double *)c->buf[c->cur ^ (int)(j & 1)];
+        t.out = (double *)c->buf[c->cur ^ (int)((j + 1) & 1)];
+        t.rates = c->rates;
+        t.st = c->st;
+        t.pchk = c->pchk;
+        t.pmass = c->pmass;
+        t.h = h;
+        t.n = c->n;
+        t.ntiles = c->dim >> TileGeom<double>::b;
+        const uint64_t cap = c->tile_blocks > 0 ? (uint64_t)c->tile_blocks : (uint64_t)tile_ref_cap();
+        const uint64_t g = t.ntiles < cap ? t.ntiles : cap;
+        t.xcd_order = (c->xcd_order && (t.ntiles % 8) == 0 && (g % 8) == 0) ? 1 : 0;
+        prof_begin(c);
+        hipLaunchKernelGGL(k_tile_ref, dim3(g), dim3(kTileThreads), 0, c->stream, t);
+        prof_end(c, "ref_tile", 16.0);
+        launch_finish(c, (uint32_t)g);
+        return;
+    }
     RefArgs a;
     a.x = (const double *)c->buf[c->cur ^ (int)(j & 1)];
     a.xn = (double *)c->buf[c->cur ^ (int)((j + 1) & 1)];

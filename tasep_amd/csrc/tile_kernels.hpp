@@ -436,4 +436,267 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pass(const TileArgs<V> a)
     }
 }
 
+// ===========================================================================
+// The reference loop (TASEP_REF) on the same FP64 tiles: one pass per step,
+// x' = x + (((k + 2k) + 2k) + k)/6 with k = h (L x)_s, bit-exact.  Each
+// state's row is accumulated in the reference's term order
+// (src/lib/solv.c:8-13 over the rows of src/lib/matgen.c:29-67): T = 0,
+// T += alpha x(entry), T += d x(s), T += beta x(s+1), then bonds k = 0..n-2
+// ascending, every product rounded before its add; the diagonal is
+// 0 - beta - alpha - out-bonds ascending.  The bond sources of the tile
+// layout (see the bit map above; FP64: e = 1, LB = 7, TB0 = 9) come in
+// exactly that order: bond 0 (next lane), bonds 1-5 (lane bonds), 6 (lane
+// bit 5 / column bit 0), 7 (column), 8 (column / tile seam), tile bonds
+// ascending, n-4 (per wave), n-3 and n-2 (other chunks).  A per-lane bond
+// that does not pull into a state adds 0 * x = +-0, which leaves T unchanged
+// (T starts at +0 and a round-to-nearest sum is -0 only if both operands
+// are); a wave- or tile-uniform one is skipped, as in the reference.
+// ===========================================================================
+struct TileRefArgs {
+    const double *x;    // state: gathered and self
+    double *out;        // next state
+    const double *rates;
+    RunState *st;
+    double *pchk, *pmass;
+    double h;
+    int n;
+    uint64_t ntiles;
+    int xcd_order;
+};
+
+__global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) {
+    using G = TileGeom<double>;
+    constexpr int E = G::E, LB = G::LB, C = G::C, TB0 = G::TB0;
+    static_assert(E == 2 && LB == 7 && TB0 == 9, "bond order below assumes the FP64 tile");
+    if (a.st->stopped) return;
+    __shared__ double tile[kTileThreads * C * E];
+    __shared__ double rr[kRateStride];
+    __shared__ double red[kTileWaves];
+    const int n = a.n;
+    if (threadIdx.x <= n) rr[threadIdx.x] = a.rates[threadIdx.x];
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int l = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const uint32_t lo = (uint32_t)l * E;
+    const double alpha = rr[0], beta = rr[n];
+    auto rb = [&](int k) { return rr[n - 1 - k]; };               // rate of bond k
+    const int tb = n - 3 - TB0;
+    const uint64_t tmask = tb > 1 ? (((uint64_t)1 << (tb - 1)) - 1) : 0;
+    const uint64_t wofs = (uint64_t)w << (n - 3);
+    const uint32_t lds0 = ((uint32_t)w * C * 64 + (uint32_t)l) * E;
+    constexpr uint32_t LCOL = 64 * E;
+    const bool w0 = w & 1, w1 = (w >> 1) & 1, w2 = (w >> 2) & 1;
+    const bool lane_hi = (l >> 5) & 1;
+    // per-lane in-flow weights and LDS sources of bonds 1..5 (lane bits q, q+1)
+    double wl[5], dl5[5];
+    uint32_t ls[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const bool on = ((l >> q) & 1) && !((l >> (q + 1)) & 1);
+        const bool off = !((l >> q) & 1) && ((l >> (q + 1)) & 1);
+        wl[q] = on ? rb(1 + q) : 0.0;
+        dl5[q] = off ? rb(1 + q) : 0.0;
+        ls[q] = on ? lds0 + ((uint32_t)E << q) : lds0;
+    }
+    const double w_b0 = (l & 1) ? 0.0 : rb(0);                    // bond 0 in-flow (odd j)
+    const double d_b0 = (l & 1) ? rb(0) : 0.0;                    // bond 0 out-flow (even j)
+    const double w_b6 = lane_hi ? rb(6) : 0.0;                    // bond 6 in-flow, columns 0, 2
+    const double d_b6 = lane_hi ? 0.0 : rb(6);                    // bond 6 out-flow, columns 1, 3
+    Kahan chk, mass;
+    const uint64_t per_xcd = a.ntiles >> 3;
+    const uint64_t stride = gridDim.x;
+    const uint64_t last = a.ntiles - 1;
+    const double h = a.h;
+
+    uint64_t it = blockIdx.x;
+    double g[C][E];
+    if (it < a.ntiles) {
+        const uint64_t base = (tile_of(it, per_xcd, a.xcd_order) << TB0) + wofs;
+#pragma unroll
+        for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), g[c]);
+    }
+    for (; it < a.ntiles; it += stride) {
+        const uint64_t T = tile_of(it, per_xcd, a.xcd_order);
+        const uint64_t wb = (T << TB0) + wofs;
+        const bool t_lo = T & 1, t_hi = (T >> (tb - 1)) & 1;
+        // ---- issue: first tile-bond stream and the next tile's own vector
+        uint64_t mm = T & ~(T >> 1) & tmask;
+        int kA = pop_bond(mm, TB0);
+        double A[C][E], B[C][E];
+        {
+            const uint64_t base = wb + (kA >= 0 ? ((uint64_t)1 << kA) : 0);
+#pragma unroll
+            for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), A[c]);
+        }
+        double gn[C][E];
+        {
+            const uint64_t itn = it + stride <= last ? it + stride : it;
+            const uint64_t base = (tile_of(itn, per_xcd, a.xcd_order) << TB0) + wofs;
+#pragma unroll
+            for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), gn[c]);
+        }
+        __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+        for (int c = 0; c < C; c++) st16(&tile[lds0 + c * LCOL], g[c]);
+        __syncthreads();
+        const uint64_t hout = ~T & (T >> 1) & tmask;
+        double Tv[C][E];
+        // ---- diagonal (reference subtraction order), then entry and d x(s)
+        {
+            double lv[E];
+            const double wa = w2 ? alpha : 0.0;
+            const uint32_t src = w2 ? lds0 - 4 * C * LCOL : lds0;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                ld16(&tile[src + c * LCOL], lv);
+#pragma unroll
+                for (int j = 0; j < E; j++) {
+                    double d = 0.0;
+                    if (j & 1) d -= beta;
+                    if (!w2) d -= alpha;
+                    if (j == 0) d -= d_b0;
+#pragma unroll
+                    for (int q = 0; q < 5; q++) d -= dl5[q];
+                    if (c & 1) d -= d_b6;
+                    if (c == 2) d -= rb(LB);
+                    if (c < 2 && t_lo) d -= rb(LB + 1);
+                    for (uint64_t m = hout; m; m &= m - 1) d -= rb(TB0 + __builtin_ctzll(m));
+                    if (!t_hi && w0) d -= rb(n - 4);
+                    if (!w0 && w1) d -= rb(n - 3);
+                    if (!w1 && w2) d -= rb(n - 2);
+                    double T0 = 0.0;
+                    T0 += wa * lv[j];
+                    T0 += d * g[c][j];
+                    Tv[c][j] = T0;
+                }
+            }
+        }
+        // ---- exit (even j reads j+1), bond 0 (odd j reads lane l+1, j = 0)
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            Tv[c][0] += beta * g[c][1];
+            double lv[E];
+            ld16(&tile[(l & 1) ? lds0 + c * LCOL : lds0 + c * LCOL + E], lv);
+            Tv[c][1] += w_b0 * lv[0];
+        }
+        // ---- bonds 1..5: lane l + 2^(k-1)
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                double lv[E];
+                ld16(&tile[ls[q] + c * LCOL], lv);
+#pragma unroll
+                for (int j = 0; j < E; j++) Tv[c][j] += wl[q] * lv[j];
+            }
+        }
+        // ---- bond 6: columns 0, 2 read lane l-32 of column c+1
+#pragma unroll
+        for (int c = 0; c < C; c += 2) {
+            double lv[E];
+            ld16(&tile[lane_hi ? lds0 + c * LCOL + LCOL - 32 * E : lds0 + c * LCOL], lv);
+#pragma unroll
+            for (int j = 0; j < E; j++) Tv[c][j] += w_b6 * lv[j];
+        }
+        // ---- bond 7: column 1 reads column 2
+#pragma unroll
+        for (int j = 0; j < E; j++) Tv[1][j] += rb(LB) * g[2][j];
+        // ---- bond 8: columns 2, 3 read columns 0, 1 of tile T+1
+        if (!t_lo) {
+            double nb[2][E];
+            ld16(a.x + wb + lo + ((uint64_t)1 << TB0), nb[0]);
+            ld16(a.x + wb + lo + ((uint64_t)1 << TB0) + (1u << LB), nb[1]);
+            const double r8 = rb(LB + 1);
+#pragma unroll
+            for (int j = 0; j < E; j++) {
+                Tv[2][j] += r8 * nb[0][j];
+                Tv[3][j] += r8 * nb[1][j];
+            }
+        }
+        // ---- tile bonds ascending, two streams in flight
+        while (kA >= 0) {
+            const int kB = pop_bond(mm, TB0);
+            {
+                const uint64_t base = wb + (kB >= 0 ? ((uint64_t)1 << kB) : 0);
+#pragma unroll
+                for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), B[c]);
+            }
+            {
+                const double r = rb(kA);
+#pragma unroll
+                for (int c = 0; c < C; c++)
+#pragma unroll
+                    for (int j = 0; j < E; j++) Tv[c][j] += r * A[c][j];
+            }
+            if (kB < 0) break;
+            kA = pop_bond(mm, TB0);
+            {
+                const uint64_t base = wb + (kA >= 0 ? ((uint64_t)1 << kA) : 0);
+#pragma unroll
+                for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), A[c]);
+            }
+            {
+                const double r = rb(kB);
+#pragma unroll
+                for (int c = 0; c < C; c++)
+#pragma unroll
+                    for (int j = 0; j < E; j++) Tv[c][j] += r * B[c][j];
+            }
+        }
+        // ---- bond n-4: chunk w+1 of tile T - 2^(tb-1), 2^(n-4) states away
+        if (t_hi && !w0) {
+            const double r = rb(n - 4);
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                double nb[E];
+                ld16(a.x + wb + ((uint64_t)1 << (n - 4)) + lo + ((uint32_t)c << LB), nb);
+#pragma unroll
+                for (int j = 0; j < E; j++) Tv[c][j] += r * nb[j];
+            }
+        }
+        // ---- bonds n-3, n-2: chunks w+1, w+2 of this tile
+        if (w0 && !w1) {
+            const double r = rb(n - 3);
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                double lv[E];
+                ld16(&tile[lds0 + c * LCOL + C * LCOL], lv);
+#pragma unroll
+                for (int j = 0; j < E; j++) Tv[c][j] += r * lv[j];
+            }
+        }
+        if (w1 && !w2) {
+            const double r = rb(n - 2);
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                double lv[E];
+                ld16(&tile[lds0 + c * LCOL + 2 * C * LCOL], lv);
+#pragma unroll
+                for (int j = 0; j < E; j++) Tv[c][j] += r * lv[j];
+            }
+        }
+        // ---- the reference update, checksum and mass
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            double o[E];
+#pragma unroll
+            for (int j = 0; j < E; j++) {
+                const double d = ref_delta(h, Tv[c][j]);
+                o[j] = g[c][j] + d;
+                chk.add(fabs(d));
+                mass.add(o[j]);
+            }
+            st16(a.out + wb + lo + ((uint32_t)c << LB), o);
+        }
+#pragma unroll
+        for (int c = 0; c < C; c++)
+#pragma unroll
+            for (int j = 0; j < E; j++) g[c][j] = gn[c][j];
+    }
+    const double bc = block_sum<kTileThreads>(chk.value(), red);
+    const double bm = block_sum<kTileThreads>(mass.value(), red);
+    store_partials(a.pchk, a.pmass, bc, bm);
+}
+
 }  // namespace tasep

@@ -77,7 +77,7 @@ def test_ref_loop_vs_reference_cli_outputs(name, gen):
 
 
 @pytest.mark.parametrize("gen", ["mf", "csr"])
-@pytest.mark.parametrize("n", [5, 12, 16])
+@pytest.mark.parametrize("n", [5, 12, 13, 14, 16])
 def test_ref_loop_bit_exact_vs_oracle(n, gen):
     r = np.random.default_rng(n).uniform(0.1, 2.0, n + 1)
     x0 = np.random.default_rng(n + 1).random(1 << n)
@@ -315,6 +315,47 @@ def test_time_variant_tiled_vs_oracle():
             e.run(0.0, 0.01, 100)
             x = e.get_x()
         assert np.max(np.abs(x - ox)) <= tol, prec
+
+
+def _ref_run(n, r, steps, x0=None, tv=None, **opts):
+    with T.Engine(n, "f64", "ref", "mf") as e:
+        for k, v in opts.items():
+            e.set_option(getattr(T, "OPT_" + k.upper()), int(v))
+        if tv is not None:
+            e.set_time_variant(tv)
+        else:
+            e.set_rates(r)
+        if x0 is None:
+            e.set_x0_uniform()
+        else:
+            e.set_x(x0)
+        info = e.run(0.0, 0.01, steps, patience=-1)
+        return e.get_x(), info
+
+
+@pytest.mark.parametrize("n", [13, 17, 20])
+def test_tiled_ref_bit_exact_vs_per_state_kernel(n):
+    """The tiled reference loop keeps the reference's term order: bitwise
+    equal to the one-lane-per-state kernel, any tile order or grid."""
+    rng = np.random.default_rng(700 + n)
+    r = rng.uniform(0.1, 2.0, n + 1)
+    x0 = rng.random(1 << n)
+    x0 /= x0.sum()
+    a, ia = _ref_run(n, r, 30, x0)
+    b, ib = _ref_run(n, r, 30, x0, tiled=0)
+    c, _ = _ref_run(n, r, 30, x0, xcd_order=0, tile_blocks=64)
+    assert _bits_equal(a, b) and _bits_equal(a, c)
+    assert ia.t == ib.t and abs(ia.last_checksum - ib.last_checksum) <= 1e-14 * ib.last_checksum
+
+
+def test_tiled_ref_time_variant_bit_exact_vs_per_state_kernel():
+    n = 14
+    rng = np.random.default_rng(71)
+    tv = dict(type=(np.arange(n + 1) % 3).astype(np.int32), amp=rng.uniform(0, 0.5, n + 1),
+              period=rng.uniform(0.5, 2, n + 1), offset=rng.uniform(0, 1, n + 1), c=np.ones(n + 1))
+    a, _ = _ref_run(n, None, 40, tv=tv)
+    b, _ = _ref_run(n, None, 40, tv=tv, tiled=0)
+    assert _bits_equal(a, b)
 
 
 # ------------------------------------------- LDS-resident whole-run kernel
