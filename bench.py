@@ -260,6 +260,17 @@ def config_3(T, ev):
     out["max_density_err_f32_vs_f64"] = float(np.max(np.abs(mom["f32"][0] - mom["f64"][0])))
     out["max_cov_err_f32_vs_f64"] = float(np.max(np.abs(mom["f32"][1] - mom["f64"][1])))
     out["speedup_f32_over_f64"] = out["f32"]["state_updates_per_s"] / out["f64"]["state_updates_per_s"]
+    # the reference's own loop (FP64, bit-faithful, first-order in h) on the
+    # same workload, for the drop-in path's throughput
+    with T.Engine(n, "f64", "ref", "mf") as e:
+        e.set_rates(r)
+        e.set_x0_uniform()
+        info, ms = timed_run(ev, e, 0.0, h, steps, patience=-1)
+        y, _, _, _ = e.moments()
+    out["ref_f64"] = {"ms": ms, "ms_per_step": ms / steps,
+                      "state_updates_per_s": steps * (1 << n) / (ms * 1e-3),
+                      "algorithmic_GBps": steps * (1 << n) * 16 / (ms * 1e-3) / 1e9,
+                      "max_density_diff_vs_rk4_f64": float(np.max(np.abs(y - mom["f64"][0])))}
     return out
 
 
