@@ -231,11 +231,18 @@ int tasep_batch_window_marginals(tasep_batch *batch, int i, double *W);
  *   TASEP_BATCH_OPT_LDS         [1]  instances with n <= 13 run in LDS;
  *                                    0 = only those with n < 10 do
  *   TASEP_BATCH_OPT_CONCURRENT  [1]  the LDS-resident instances run on a
- *                                    second stream beside the shared passes */
+ *                                    second stream beside the shared passes
+ *   TASEP_BATCH_OPT_SCHEDULE    [0]  shared passes: 1 one lane per state,
+ *                                    2 tiles of 2^13 (FP32) / 2^12 (FP64)
+ *                                    states in LDS, 0 = timed on the first
+ *                                    run and the faster kept (get_option
+ *                                    then reports the choice) */
 #define TASEP_BATCH_OPT_GRAPH_STEPS 1
 #define TASEP_BATCH_OPT_LDS         2
 #define TASEP_BATCH_OPT_CONCURRENT  3
+#define TASEP_BATCH_OPT_SCHEDULE    4
 int tasep_batch_set_option(tasep_batch *batch, int key, int64_t value);
+int tasep_batch_get_option(tasep_batch *batch, int key, int64_t *value);
 int tasep_batch_get_stream(tasep_batch *batch, void **stream);
 
 const char *tasep_strerror(int code);

@@ -35,10 +35,10 @@ EXPORTS = [
     "tasep_batch_set_time_variant", "tasep_batch_set_x0_uniform", "tasep_batch_set_x",
     "tasep_batch_get_x", "tasep_batch_run", "tasep_batch_moments",
     "tasep_batch_set_option", "tasep_batch_get_stream", "tasep_window_marginals",
-    "tasep_batch_window_marginals",
+    "tasep_batch_window_marginals", "tasep_batch_get_option",
 ]
 OPT_TILED, OPT_XCD_ORDER, OPT_RK4_STAGED, OPT_TILE_BLOCKS, OPT_LDS, OPT_LDS_THREADS = 1, 2, 3, 4, 5, 6
-BATCH_OPT_GRAPH_STEPS, BATCH_OPT_LDS, BATCH_OPT_CONCURRENT = 1, 2, 3
+BATCH_OPT_GRAPH_STEPS, BATCH_OPT_LDS, BATCH_OPT_CONCURRENT, BATCH_OPT_SCHEDULE = 1, 2, 3, 4
 
 
 class TasepError(RuntimeError):
@@ -123,6 +123,7 @@ def load_library():
                                   C.c_int, C.POINTER(_RunInfo)]
     L.tasep_batch_moments.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     L.tasep_batch_set_option.argtypes = [vp, C.c_int, C.c_int64]
+    L.tasep_batch_get_option.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.tasep_batch_get_stream.argtypes = [vp, C.POINTER(vp)]
     L.tasep_window_marginals.argtypes = [vp, _f64p]
     L.tasep_batch_window_marginals.argtypes = [vp, C.c_int, _f64p]
@@ -344,6 +345,11 @@ class Batch:
 
     def set_option(self, key: int, value: int):
         _check(load_library().tasep_batch_set_option(self._h, key, value))
+
+    def get_option(self, key: int) -> int:
+        v = C.c_int64()
+        _check(load_library().tasep_batch_get_option(self._h, key, C.byref(v)))
+        return int(v.value)
 
     def run(self, t0, h, steps, thr=1e-7, patience=-1):
         arr = (_RunInfo * self.count)()

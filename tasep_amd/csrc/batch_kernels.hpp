@@ -205,4 +205,195 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_finish(const BatchInst 
     if (threadIdx.x == 0) advance_state(st, chk, mass);
 }
 
+// ===========================================================================
+// Tile form of the batched pass.  A 1024-lane workgroup takes one tile of
+// 2^TL consecutive states of one instance (TL = 13 FP32, 12 FP64; 32 KiB of
+// LDS) and lane t holds local states t + 1024 j.  Bonds inside the tile come
+// from LDS (bonds < 6 branch-free per lane, bonds 6..TL-2 wave-uniform);
+// the bonds that cross tiles are uniform over the workgroup: the seam bond
+// TL-1 (local bit TL-1, tile bit 0: half a tile of T+1), the tile bonds
+// (tile T + 2^(k-TL), two streams in flight) and the entry (tile T - 2^(n-1-TL)),
+// each a coalesced stream read only when it pulls into the tile.
+// ===========================================================================
+constexpr int kBatchTileThreads = 1024;
+template <class V> struct BatchTileGeom;
+template <> struct BatchTileGeom<float> { static constexpr int TL = 13, J = 8; };
+template <> struct BatchTileGeom<double> { static constexpr int TL = 12, J = 4; };
+
+template <class V>
+__global__ __launch_bounds__(kBatchTileThreads, 8) void k_batch_tile(const BatchArgs<V> a) {
+    using A = V;
+    constexpr int TL = BatchTileGeom<V>::TL, J = BatchTileGeom<V>::J;
+    constexpr uint32_t TS = 1u << TL;
+    constexpr int NT = kBatchTileThreads;
+    __shared__ V tile[TS];
+    __shared__ V rs[kRateStride];
+    __shared__ double red[NT / kWave];
+    const BatchBlock bb = a.blocks[blockIdx.x];
+    if (bb.inst == 0xffffffffu) return;
+    const BatchInst in = a.inst[bb.inst];
+    RunState *st = a.st + in.id;
+    if (st->stopped) return;
+    const int n = in.n, p = a.pass, t = threadIdx.x;
+    const double h = st->h;
+    if (t <= n) {
+        double r;
+        if (in.tv) {
+            const int o = in.tv_off + t;
+            const double t0 = st->t;
+            const double tq = p == 0 ? t0 : (p < 3 ? t0 + 0.5 * h : t0 + h);
+            r = tv_rate(a.tv_type[o], a.tv_amp[o], a.tv_period[o], a.tv_offset[o], a.tv_c[o], tq);
+        } else {
+            r = a.rates[in.rate_off + t];
+        }
+        rs[t] = (V)r;
+    }
+    V *x = a.x + in.off, *u = a.u + in.off, *v = a.v + in.off, *kacc = a.acc + in.off;
+    const V *g = p == 0 ? x : (p == 2 ? v : u);
+    V *dst = p == 1 ? v : u;
+    const uint32_t T = bb.blk;
+    const uint32_t base = T << TL;
+    const int ntb = n - TL;                                  // tile-index bits (>= 1)
+    // ---- own values, self term and the first uniform streams
+    A own[J], xs[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        own[j] = (A)g[base + t + j * NT];
+        xs[j] = p == 0 ? own[j] : (A)x[base + t + j * NT];
+    }
+    const bool seam_on = !(T & 1u);                          // bond TL-1 pulls from tile T+1
+    A seam[J / 2];
+    if (seam_on) {
+#pragma unroll
+        for (int j = 0; j < J / 2; j++) seam[j] = (A)g[base + TS + t + j * NT];
+    }
+    const uint32_t tmask = ntb > 1 ? ((1u << (ntb - 1)) - 1) : 0u;  // tile bonds TL..n-2
+    uint32_t mm = T & ~(T >> 1) & tmask;
+    int kA = mm ? TL + __builtin_ctz(mm) : -1;
+    if (mm) mm &= mm - 1;
+    A sA[J], sB[J];
+    if (kA >= 0) {
+#pragma unroll
+        for (int j = 0; j < J; j++) sA[j] = (A)g[base + (1u << kA) + t + j * NT];
+    }
+#pragma unroll
+    for (int j = 0; j < J; j++) tile[t + j * NT] = (V)own[j];
+    __syncthreads();
+    BatchRates<A> q;
+    q.alpha = (A)rs[0];
+    q.beta = (A)rs[n];
+#pragma unroll
+    for (int k = 0; k < kLaneBits; k++) q.bk[k] = (A)rs[n - 1 - k];
+    // ---- uniform parts of the outflow: tile bonds, entry
+    const bool t_top = (T >> (ntb - 1)) & 1u;
+    const uint32_t hout = ~T & (T >> 1) & tmask;
+    A d_tile = t_top ? (A)0 : q.alpha;
+    for (uint32_t m = hout; m; m &= m - 1) d_tile += (A)rs[n - 1 - (TL + __builtin_ctz(m))];
+    const A r_seam = (A)rs[n - TL];                          // bond TL-1
+    // ---- in-tile rows
+    A L[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const uint32_t loc = t + j * NT;
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(loc & ~63u));
+        A d = d_tile + ((loc & 1) ? q.beta : (A)0);
+#pragma unroll
+        for (int k = 0; k < kLaneBits; k++)
+            if (((loc >> k) & 3u) == 2u) d += q.bk[k];
+        for (int k = kLaneBits; k < TL - 1; k++)
+            if (((hi >> k) & 3u) == 2u) d += (A)rs[n - 1 - k];
+        if (!(loc & (TS >> 1)) && (T & 1u)) d += r_seam;    // seam out-flow
+        A acc = -d * own[j];
+        acc = fma((loc & 1) ? (A)0 : q.beta, (A)tile[loc ^ 1u], acc);
+#pragma unroll
+        for (int k = 0; k < kLaneBits; k++)
+            acc = fma(((loc >> k) & 3u) == 1u ? q.bk[k] : (A)0, (A)tile[loc ^ (3u << k)], acc);
+        for (int k = kLaneBits; k < TL - 1; k++)
+            if (((hi >> k) & 3u) == 1u) acc = fma((A)rs[n - 1 - k], (A)tile[loc + (1u << k)], acc);
+        L[j] = acc;
+    }
+    // ---- cross-tile in-flow: seam, tile bonds (two streams in flight), entry
+    if (seam_on) {
+#pragma unroll
+        for (int j = J / 2; j < J; j++) L[j] = fma(r_seam, seam[j - J / 2], L[j]);
+    }
+    while (kA >= 0) {
+        const int kB = mm ? TL + __builtin_ctz(mm) : -1;
+        if (mm) mm &= mm - 1;
+        if (kB >= 0) {
+#pragma unroll
+            for (int j = 0; j < J; j++) sB[j] = (A)g[base + (1u << kB) + t + j * NT];
+        }
+        {
+            const A r = (A)rs[n - 1 - kA];
+#pragma unroll
+            for (int j = 0; j < J; j++) L[j] = fma(r, sA[j], L[j]);
+        }
+        if (kB < 0) break;
+        kA = mm ? TL + __builtin_ctz(mm) : -1;
+        if (mm) mm &= mm - 1;
+        if (kA >= 0) {
+#pragma unroll
+            for (int j = 0; j < J; j++) sA[j] = (A)g[base + (1u << kA) + t + j * NT];
+        }
+        {
+            const A r = (A)rs[n - 1 - kB];
+#pragma unroll
+            for (int j = 0; j < J; j++) L[j] = fma(r, sB[j], L[j]);
+        }
+    }
+    if (t_top) {
+        const uint32_t eo = base - (1u << (n - 1));
+#pragma unroll
+        for (int j = 0; j < J; j++) L[j] = fma(q.alpha, (A)g[eo + t + j * NT], L[j]);
+    }
+    // ---- epilogue: Horner or stored stages
+    Kahan ck, km;
+    if (!in.tv) {
+        const A cs = p == 0 ? (A)(0.25 * h) : (p == 1 ? (A)(h / 3.0) : (p == 2 ? (A)(0.5 * h) : (A)h));
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const uint32_t s = base + t + j * NT;
+            if (p < 3) {
+                dst[s] = (V)fma(cs, L[j], xs[j]);
+            } else {
+                const A d = cs * L[j];
+                const V xn = (V)(xs[j] + d);
+                x[s] = xn;
+                ck.add(fabs((double)d));
+                km.add((double)xn);
+            }
+        }
+    } else {
+        const A hv = (A)h;
+        const A cy = p == 2 ? (A)1 : (A)0.5;
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const uint32_t s = base + t + j * NT;
+            const A kk = hv * L[j];
+            if (p == 0) {
+                kacc[s] = (V)kk;
+                dst[s] = (V)fma(cy, kk, xs[j]);
+            } else if (p < 3) {
+                kacc[s] = (V)((A)kacc[s] + (A)2 * kk);
+                dst[s] = (V)fma(cy, kk, xs[j]);
+            } else {
+                const A d = ((A)kacc[s] + kk) / (A)6;
+                const V xn = (V)(xs[j] + d);
+                x[s] = xn;
+                ck.add(fabs((double)d));
+                km.add((double)xn);
+            }
+        }
+    }
+    if (p == 3) {
+        const double bc = block_sum<NT>(ck.value(), red);
+        const double bm = block_sum<NT>(km.value(), red);
+        if (t == 0) {
+            a.pchk[in.part_off + bb.blk] = bc;
+            a.pmass[in.part_off + bb.blk] = bm;
+        }
+    }
+}
+
 }  // namespace tasep

@@ -58,6 +58,8 @@ struct tasep_batch {
     // options
     int graph_steps = 32;       // steps per captured graph (0: direct launches)
     int use_lds = 1;
+    int schedule = 0;           // shared passes: 0 autotuned, 1 one lane per state, 2 tiles
+    int chosen = 1;             // schedule the current block table is built for
     int concurrent = 1;
     hipGraphExec_t graph = nullptr;
     int graph_built_steps = 0;
@@ -81,8 +83,23 @@ void drop_graph(tasep_batch *b) {
     b->graph_built_steps = 0;
 }
 
-// Instance tables: LDS class and streamed class, XCD-aware block table.
-int build_tables(tasep_batch *b) {
+// States per workgroup of the shared passes under schedule `sched`.
+uint32_t states_per_block(const tasep_batch *b, int sched) {
+    if (sched == 2) return 1u << (b->prec == TASEP_F32 ? BatchTileGeom<float>::TL : BatchTileGeom<double>::TL);
+    return kBatchThreads * (b->prec == TASEP_F32 ? BatchGeom<float>::E : BatchGeom<double>::E);
+}
+
+// The tile schedule needs at least two tiles per streamed instance.
+bool tiles_fit(const tasep_batch *b) {
+    const uint32_t per = states_per_block(b, 2);
+    for (int i = 0; i < b->count; i++)
+        if (!lds_class(b, i) && ((uint64_t)1 << b->n[i]) < 2 * (uint64_t)per) return false;
+    return true;
+}
+
+// Instance tables: LDS class and streamed class, XCD-aware block table for
+// schedule `sched`.
+int build_tables(tasep_batch *b, int sched) {
     b->lds_ids.clear();
     b->str_ids.clear();
     std::vector<int> lds_tv;
@@ -112,7 +129,8 @@ int build_tables(tasep_batch *b) {
     // runs on XCD w % 8, so instance j's blocks go to the XCD with the
     // fewest blocks so far (largest instances first) at positions
     // 8 * k + xcd: each instance's vectors stay in one XCD's L2.
-    const uint32_t per = kBatchThreads * (b->prec == TASEP_F32 ? BatchGeom<float>::E : BatchGeom<double>::E);
+    const uint32_t per = states_per_block(b, sched);
+    b->chosen = sched;
     std::vector<BatchInst> bt(b->str_ids.size());
     uint32_t slots = 0;
     for (size_t j = 0; j < bt.size(); j++) {
@@ -186,9 +204,15 @@ BatchArgs<V> batch_args(tasep_batch *b, int pass) {
     return a;
 }
 
-// The streamed instances' passes of one step and their run-state update.
-void enqueue_stream_step(tasep_batch *b, hipStream_t s) {
-    for (int p = 0; p < 4; p++) {
+void launch_pass(tasep_batch *b, int p, hipStream_t s) {
+    if (b->chosen == 2) {
+        if (b->prec == TASEP_F32)
+            hipLaunchKernelGGL(k_batch_tile<float>, dim3(b->nblocks), dim3(kBatchTileThreads), 0, s,
+                               batch_args<float>(b, p));
+        else
+            hipLaunchKernelGGL(k_batch_tile<double>, dim3(b->nblocks), dim3(kBatchTileThreads), 0, s,
+                               batch_args<double>(b, p));
+    } else {
         if (b->prec == TASEP_F32)
             hipLaunchKernelGGL(k_batch_pass<float>, dim3(b->nblocks), dim3(kBatchThreads), 0, s,
                                batch_args<float>(b, p));
@@ -196,6 +220,35 @@ void enqueue_stream_step(tasep_batch *b, hipStream_t s) {
             hipLaunchKernelGGL(k_batch_pass<double>, dim3(b->nblocks), dim3(kBatchThreads), 0, s,
                                batch_args<double>(b, p));
     }
+}
+
+// Autotuner of the shared passes: time a side-effect-free pass (pass 2 of
+// the step writes only the work vectors, which the next step's first pass
+// rewrites) under both schedules and keep the faster.  Needs valid run
+// states (h, stopped) on the device.
+int autotune(tasep_batch *b) {
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    float ms[3] = {0, 0, 0};
+    for (int sc = 1; sc <= 2; sc++) {
+        int rc = build_tables(b, sc);
+        if (rc) return rc;
+        launch_pass(b, 1, b->stream);
+        HIP_OK(hipEventRecord(e0, b->stream));
+        for (int r = 0; r < 3; r++) launch_pass(b, 1, b->stream);
+        HIP_OK(hipEventRecord(e1, b->stream));
+        HIP_OK(hipEventSynchronize(e1));
+        HIP_OK(hipEventElapsedTime(&ms[sc], e0, e1));
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return build_tables(b, ms[2] < ms[1] ? 2 : 1);
+}
+
+// The streamed instances' passes of one step and their run-state update.
+void enqueue_stream_step(tasep_batch *b, hipStream_t s) {
+    for (int p = 0; p < 4; p++) launch_pass(b, p, s);
     hipLaunchKernelGGL(k_batch_finish, dim3((unsigned)b->str_ids.size()), dim3(kBatchThreads), 0, s, b->btab,
                        b->st, b->pchk, b->pmass);
 }
@@ -419,6 +472,18 @@ int tasep_batch_window_marginals(tasep_batch *b, int i, double *W) {
     return device_windows(b->stream, x, b->n[i], b->prec, &b->mom_part, &b->mom_out, W);
 }
 
+int tasep_batch_get_option(tasep_batch *b, int key, int64_t *value) {
+    if (!b || !value) return fail(TASEP_ERR_ARG, "NULL argument");
+    switch (key) {
+    case TASEP_BATCH_OPT_GRAPH_STEPS: *value = b->graph_steps; break;
+    case TASEP_BATCH_OPT_LDS: *value = b->use_lds; break;
+    case TASEP_BATCH_OPT_CONCURRENT: *value = b->concurrent; break;
+    case TASEP_BATCH_OPT_SCHEDULE: *value = b->tables_dirty ? b->schedule : b->chosen; break;
+    default: return fail(TASEP_ERR_ARG, "unknown batch option");
+    }
+    return TASEP_OK;
+}
+
 int tasep_batch_set_option(tasep_batch *b, int key, int64_t value) {
     if (!b) return fail(TASEP_ERR_ARG, "NULL batch");
     switch (key) {
@@ -432,6 +497,11 @@ int tasep_batch_set_option(tasep_batch *b, int key, int64_t value) {
         b->tables_dirty = true;
         break;
     case TASEP_BATCH_OPT_CONCURRENT: b->concurrent = value != 0; break;
+    case TASEP_BATCH_OPT_SCHEDULE:
+        if (value < 0 || value > 2) return fail(TASEP_ERR_ARG, "schedule: 0 (autotuned), 1 or 2");
+        b->schedule = (int)value;
+        b->tables_dirty = true;
+        break;
     default: return fail(TASEP_ERR_ARG, "unknown batch option");
     }
     return TASEP_OK;
@@ -451,10 +521,6 @@ int tasep_batch_run(tasep_batch *b, double t0, double h, uint64_t steps, double 
         HIP_OK(hipMemcpy(b->tv_par, b->h_tv_par.data(), 4 * m * sizeof(double), hipMemcpyHostToDevice));
         b->rates_dirty = false;
     }
-    if (b->tables_dirty) {
-        int rc = build_tables(b);
-        if (rc) return rc;
-    }
     for (int i = 0; i < b->count; i++) {
         RunState &s = b->st_host[i];
         memset(&s, 0, sizeof s);
@@ -466,6 +532,16 @@ int tasep_batch_run(tasep_batch *b, double t0, double h, uint64_t steps, double 
     }
     HIP_OK(hipMemcpyAsync(b->st, b->st_host, (size_t)b->count * sizeof(RunState), hipMemcpyHostToDevice,
                           b->stream));
+    if (b->tables_dirty) {
+        int sched = b->schedule;
+        // the streamed set decides whether tiles fit (build once to know it)
+        int rc = build_tables(b, 1);
+        if (rc) return rc;
+        if (!tiles_fit(b)) sched = 1;
+        if (sched == 0 && !b->str_ids.empty()) rc = autotune(b);
+        else rc = build_tables(b, sched == 0 ? 1 : sched);
+        if (rc) return rc;
+    }
     // LDS-resident instances: whole run, on the second stream when concurrent
     hipStream_t ls = b->concurrent ? b->stream2 : b->stream;
     if (!b->lds_ids.empty()) {

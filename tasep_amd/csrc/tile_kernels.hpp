@@ -46,6 +46,10 @@
 namespace tasep {
 
 constexpr int kTileThreads = 512;
+// Waves per SIMD the tile passes are compiled for (register budget 512/W).
+#ifndef TASEP_TILE_WAVES
+#define TASEP_TILE_WAVES 2
+#endif
 constexpr int kTileWaves = kTileThreads / kWave;
 
 template <class V> struct TileGeom;
@@ -179,7 +183,7 @@ __device__ __forceinline__ int pop_bond(uint64_t &m, int LB0) {
 }
 
 template <class V, bool TWO, int EPI, bool SELF>
-__global__ __launch_bounds__(kTileThreads) void k_tile_pass(const TileArgs<V> a) {
+__global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(const TileArgs<V> a) {
     using G = TileGeom<V>;
     constexpr int E = G::E, e = G::e, LB = G::LB, C = G::C, TB0 = G::TB0;
     if (a.st->stopped) return;

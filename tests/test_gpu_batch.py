@@ -45,17 +45,22 @@ def _oracle(n, spec, steps, h=0.01):
 NS = [2, 5, 8, 9, 10, 12, 13, 14, 15, 16, 17]
 
 
+@pytest.mark.parametrize("schedule", [0, 1, 2])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
-def test_batch_matches_oracle_per_instance(prec, tol):
+def test_batch_matches_oracle_per_instance(prec, tol, schedule):
+    """Every instance against its own oracle run, under the autotuned (0),
+    one-lane-per-state (1) and tile (2) schedules of the shared passes."""
     steps = 40
-    b, spec = _setup(NS + NS[::-1], 512, prec)
+    b, spec = _setup(NS + NS[::-1], 512, prec, schedule=schedule)
     with b:
         infos = b.run(0.0, 0.01, steps)
         xs = [b.get_x(i) for i in range(b.count)]
+        chosen = b.get_option(T.BATCH_OPT_SCHEDULE)
+    assert chosen in (1, 2) and (schedule == 0 or chosen == schedule)
     for i, n in enumerate(b.ns):
         ox = _oracle(n, spec[i], steps)
         err = np.max(np.abs(xs[i] - ox))
-        print(f"instance {i} n={n} {spec[i][0]} {prec}: max|x - x_oracle| = {err:.3e}")
+        print(f"schedule {chosen} instance {i} n={n} {spec[i][0]} {prec}: max|x - x_oracle| = {err:.3e}")
         assert err <= tol, (i, n, spec[i][0], err)
         assert infos[i].steps_done == steps and not infos[i].stopped
         assert infos[i].t == pytest.approx(0.01 * steps, abs=1e-12)
@@ -87,14 +92,17 @@ def test_batch_graph_and_streams_do_not_change_results():
     steps = 21   # two graph replays of 8 steps and 5 direct steps
     runs = []
     for opts in (dict(graph_steps=0, concurrent=0), dict(graph_steps=8, concurrent=1),
-                 dict(graph_steps=8, concurrent=0)):
+                 dict(graph_steps=8, concurrent=0), dict(graph_steps=8, schedule=2),
+                 dict(graph_steps=0, schedule=2)):
         b, _ = _setup(ns, 3, "f32", **opts)
         with b:
             b.run(0.0, 0.01, steps)
             runs.append([b.get_x(i) for i in range(len(ns))])
-    for r in runs[1:]:
+    for r in runs[1:3]:
         for a, c in zip(runs[0], r):
             assert np.array_equal(a, c)
+    for a, c in zip(runs[3], runs[4]):     # tiles: graph replay changes nothing
+        assert np.array_equal(a, c)
 
 
 def test_batch_streamed_small_instances_match_lds():
