@@ -158,9 +158,10 @@ int tasep_step_profile(tasep_ctx *ctx, double t0, double h, int reps,
                        tasep_kernel_time *out, int max, int *count);
 
 /* Engine options (no reference counterpart; defaults in brackets):
- *   TASEP_OPT_TILED        [1] tiled matrix-free passes when n is large
- *                              enough (n >= 12 FP32, n >= 11 FP64); 0 = one
- *                              lane per state everywhere
+ *   TASEP_OPT_TILED        [1] tiled matrix-free passes: 1 for n >= 18
+ *                              (RK4) / n >= 19 (reference loop), 2 wherever
+ *                              a lattice has two tiles (n >= 14 FP32,
+ *                              n >= 13 FP64), 0 = one lane per state
  *   TASEP_OPT_XCD_ORDER    [1] tiles walked in contiguous eighths per XCD
  *   TASEP_OPT_RK4_STAGED   [0] RK4 with four stored stages even for constant
  *                              rates (default: the Horner form of the same
@@ -174,13 +175,18 @@ int tasep_step_profile(tasep_ctx *ctx, double t0, double h, int reps,
  *                              0 never, 1 for n <= 12, 2 wherever it fits
  *                              (n <= 13)
  *   TASEP_OPT_LDS_THREADS  [0] workgroup size of those runs (power of two in
- *                              [64, 1024]); 0 = 2^n clamped to that range */
+ *                              [64, 1024]); 0 = 2^n clamped to that range
+ *   TASEP_OPT_GRAPH        [0] runs of at least two chunks (a chunk is
+ *                              ~2^26 state-updates, 4..4096 steps) replay a
+ *                              HIP graph of one chunk's launches (measured
+ *                              no faster than direct launches on MI355X) */
 #define TASEP_OPT_TILED        1
 #define TASEP_OPT_XCD_ORDER    2
 #define TASEP_OPT_RK4_STAGED   3
 #define TASEP_OPT_TILE_BLOCKS  4
 #define TASEP_OPT_LDS          5
 #define TASEP_OPT_LDS_THREADS  6
+#define TASEP_OPT_GRAPH        7
 int tasep_set_option(tasep_ctx *ctx, int key, int64_t value);
 
 /* The context's HIP stream (hipStream_t) and device pointer of the current

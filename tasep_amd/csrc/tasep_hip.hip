@@ -44,7 +44,8 @@ struct tasep_ctx {
     double *mom_part = nullptr, *mom_out = nullptr;
     double host_rates[kRateStride] = {0};
     // options (tasep_set_option)
-    bool tiled = true;          // tiled matrix-free passes where n allows
+    int tiled = 1;              // tiled matrix-free passes: 0 never, 1 n >= 18 (19 REF), 2 where they fit
+    bool graphs = false;        // long runs replay a captured graph of steps
     bool xcd_order = true;      // XCD-aware tile order
     bool force_staged = false;  // staged RK4 even for constant rates
     int tile_blocks = 0;        // persistent grid of the tiled passes (0: by occupancy)
@@ -533,9 +534,14 @@ static void rk4_staged_tile(tasep_ctx *c, double h, uint64_t j) {
     launch_tile<V, true, EPI_RK4>(c, a, "rk4_s4", 5 * e);
 }
 
+// Tiles need n >= b + 1 (two tiles); below n = 18 the one-lane-per-state
+// passes are faster (too few tiles to fill 256 CUs).
+constexpr int kTileAutoMinN = 18, kTileRefAutoMinN = 19;
+
 template <class V>
 static bool use_tile(const tasep_ctx *c) {
-    return c->tiled && c->gen == TASEP_MF && c->n >= TileGeom<V>::b + 1;
+    if (!c->tiled || c->gen != TASEP_MF || c->n < TileGeom<V>::b + 1) return false;
+    return c->tiled == 2 || c->n >= kTileAutoMinN;
 }
 
 static int tile_ref_cap() {
@@ -555,7 +561,8 @@ static int tile_ref_cap() {
 }
 
 static bool use_tile_ref(const tasep_ctx *c) {
-    return c->tiled && c->gen == TASEP_MF && c->n >= TileGeom<double>::b + 1;
+    if (!c->tiled || c->gen != TASEP_MF || c->n < TileGeom<double>::b + 1) return false;
+    return c->tiled == 2 || c->n >= kTileRefAutoMinN;
 }
 
 static void ref_step(tasep_ctx *c, double h, uint64_t j) {
@@ -773,6 +780,42 @@ int tasep_run_resume(tasep_ctx *c, double t0, double h, uint64_t steps, double t
     if (chunk < 4) chunk = 4;
     if (chunk > 4096) chunk = 4096;
     uint64_t j = 0;
+    // Long runs replay a captured HIP graph of `chunk` steps (an even number,
+    // so the ping-pong buffers keep their parity); the remainder and short
+    // runs are launched directly.
+    if (c->graphs && chunk % 2 == 0 && steps >= 2 * chunk) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t exec = nullptr;
+        const uint32_t p0 = c->pending;
+        HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (uint64_t k = 0; k < chunk; k++) enqueue_step(c, h, k);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        const uint32_t per = c->pending - p0;   // deferred reductions per replay
+        c->pending = p0;
+        if (e != hipSuccess) return fail(TASEP_ERR_HIP, "hipStreamEndCapture", e);
+        e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(TASEP_ERR_HIP, "hipGraphInstantiate", e);
+        bool stopped = false;
+        while (j + chunk <= steps && !stopped) {
+            e = hipGraphLaunch(exec, c->stream);
+            if (e != hipSuccess) {
+                hipGraphExecDestroy(exec);
+                return fail(TASEP_ERR_HIP, "hipGraphLaunch", e);
+            }
+            c->pending += per;
+            j += chunk;
+            if (watch && j < steps) {
+                HIP_OK(hipMemcpyAsync(&c->st_host->stopped, &c->st->stopped, sizeof(int32_t),
+                                      hipMemcpyDeviceToHost, c->stream));
+                HIP_OK(hipStreamSynchronize(c->stream));
+                stopped = c->st_host->stopped != 0;
+            }
+        }
+        HIP_OK(hipStreamSynchronize(c->stream));
+        hipGraphExecDestroy(exec);
+        if (stopped) return finish_run(c, info);
+    }
     while (j < steps) {
         const uint64_t end = j + chunk < steps ? j + chunk : steps;
         for (; j < end; j++) enqueue_step(c, h, j);
@@ -790,9 +833,13 @@ int tasep_run_resume(tasep_ctx *c, double t0, double h, uint64_t steps, double t
 int tasep_set_option(tasep_ctx *c, int key, int64_t value) {
     if (!c) return fail(TASEP_ERR_ARG, "NULL context");
     switch (key) {
-    case TASEP_OPT_TILED: c->tiled = value != 0; break;
+    case TASEP_OPT_TILED:
+        if (value < 0 || value > 2) return fail(TASEP_ERR_ARG, "tiled: 0, 1 or 2");
+        c->tiled = (int)value;
+        break;
     case TASEP_OPT_XCD_ORDER: c->xcd_order = value != 0; break;
     case TASEP_OPT_RK4_STAGED: c->force_staged = value != 0; break;
+    case TASEP_OPT_GRAPH: c->graphs = value != 0; break;
     case TASEP_OPT_LDS:
         if (value < 0 || value > 2) return fail(TASEP_ERR_ARG, "LDS mode: 0, 1 or 2");
         c->lds = (int)value;

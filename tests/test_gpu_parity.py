@@ -266,7 +266,7 @@ def _rk4_run(n, prec, r, steps, **opts):
 @pytest.mark.parametrize("n", [12, 13, 14, 15, 17])
 def test_tiled_f64_matches_per_state_kernel(n):
     r = np.random.default_rng(40 + n).uniform(0.2, 1.5, n + 1)
-    xt = _rk4_run(n, "f64", r, 60, tiled=1)
+    xt = _rk4_run(n, "f64", r, 60, tiled=2)
     xs = _rk4_run(n, "f64", r, 60, tiled=0)
     assert np.max(np.abs(xt - xs)) <= 1e-14
 
@@ -277,7 +277,7 @@ def test_tiled_f32_matches_f64_oracle(n):
     x0 = np.zeros(1 << n)
     x0[0] = 1.0
     ox, _ = oracle.run_rk4(n, r, x0, 0.0, 0.01, 60)
-    xt = _rk4_run(n, "f32", r, 60, tiled=1)
+    xt = _rk4_run(n, "f32", r, 60, tiled=2)
     err = np.max(np.abs(xt - ox))
     print(f"n={n} FP32 tiled max|x - x_f64| = {err:.3e}")
     assert err <= 1e-6
@@ -287,13 +287,13 @@ def test_tiled_f32_matches_f64_oracle(n):
 def test_xcd_order_does_not_change_results():
     n = 17
     r = np.random.default_rng(7).uniform(0.2, 1.5, n + 1)
-    a = _rk4_run(n, "f32", r, 30, xcd_order=1)
-    b = _rk4_run(n, "f32", r, 30, xcd_order=0)
-    c = _rk4_run(n, "f32", r, 30, tile_blocks=64)
+    a = _rk4_run(n, "f32", r, 30, tiled=2, xcd_order=1)
+    b = _rk4_run(n, "f32", r, 30, tiled=2, xcd_order=0)
+    c = _rk4_run(n, "f32", r, 30, tiled=2, tile_blocks=64)
     assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
-@pytest.mark.parametrize("prec,tiled", [("f64", 1), ("f64", 0), ("f32", 1)])
+@pytest.mark.parametrize("prec,tiled", [("f64", 2), ("f64", 0), ("f32", 2)])
 def test_horner_equals_staged_rk4(prec, tiled):
     n = 15
     r = np.random.default_rng(8).uniform(0.2, 1.5, n + 1)
@@ -310,6 +310,7 @@ def test_time_variant_tiled_vs_oracle():
     ox, _ = oracle.run_rk4(n, None, oracle.uniform_x0(n), 0.0, 0.01, 100, tv=tv)
     for prec, tol in (("f64", 1e-12), ("f32", 2e-7)):
         with T.Engine(n, prec, "rk4", "mf") as e:
+            e.set_option(T.OPT_TILED, 2)
             e.set_time_variant(tv)
             e.set_x0_uniform()
             e.run(0.0, 0.01, 100)
@@ -341,9 +342,9 @@ def test_tiled_ref_bit_exact_vs_per_state_kernel(n):
     r = rng.uniform(0.1, 2.0, n + 1)
     x0 = rng.random(1 << n)
     x0 /= x0.sum()
-    a, ia = _ref_run(n, r, 30, x0)
+    a, ia = _ref_run(n, r, 30, x0, tiled=2)
     b, ib = _ref_run(n, r, 30, x0, tiled=0)
-    c, _ = _ref_run(n, r, 30, x0, xcd_order=0, tile_blocks=64)
+    c, _ = _ref_run(n, r, 30, x0, tiled=2, xcd_order=0, tile_blocks=64)
     assert _bits_equal(a, b) and _bits_equal(a, c)
     assert ia.t == ib.t and abs(ia.last_checksum - ib.last_checksum) <= 1e-14 * ib.last_checksum
 
@@ -353,9 +354,40 @@ def test_tiled_ref_time_variant_bit_exact_vs_per_state_kernel():
     rng = np.random.default_rng(71)
     tv = dict(type=(np.arange(n + 1) % 3).astype(np.int32), amp=rng.uniform(0, 0.5, n + 1),
               period=rng.uniform(0.5, 2, n + 1), offset=rng.uniform(0, 1, n + 1), c=np.ones(n + 1))
-    a, _ = _ref_run(n, None, 40, tv=tv)
+    a, _ = _ref_run(n, None, 40, tv=tv, tiled=2)
     b, _ = _ref_run(n, None, 40, tv=tv, tiled=0)
     assert _bits_equal(a, b)
+
+
+def test_graph_replay_does_not_change_results():
+    """n=18: chunks of 256 steps are replayed from a captured graph; 600 steps
+    = 2 replays + 88 direct steps, bitwise equal to direct launches."""
+    n = 18
+    rng = np.random.default_rng(81)
+    r = rng.uniform(0.2, 1.5, n + 1)
+    tv = dict(type=np.zeros(n + 1, np.int32), amp=rng.uniform(0, 0.5, n + 1),
+              period=rng.uniform(0.5, 2, n + 1), offset=np.zeros(n + 1), c=np.ones(n + 1))
+    for prec, integ, kind in (("f64", "ref", "r"), ("f32", "rk4", "r"), ("f32", "rk4", "tv"),
+                              ("f64", "ref", "tv")):
+        out = []
+        for graph in (1, 0):
+            with T.Engine(n, prec, integ, "mf") as e:
+                e.set_option(T.OPT_GRAPH, graph)
+                if kind == "tv":
+                    e.set_time_variant(tv)
+                else:
+                    e.set_rates(r)
+                e.set_x0_state(0)
+                info = e.run(0.0, 0.01, 600, patience=-1)
+                out.append((e.get_x(), info))
+        assert np.array_equal(out[0][0], out[1][0]), (prec, integ, kind)
+        assert out[0][1].t == out[1][1].t and out[0][1].steps_done == out[1][1].steps_done == 600
+    # early exit inside a replayed chunk: alpha = beta = 1/2, gamma = 1, uniform
+    with T.Engine(n, "f64", "ref", "mf") as e:
+        e.set_rates(np.array([0.5] + [1.0] * (n - 1) + [0.5]))
+        e.set_x0_uniform()
+        info = e.run(0.0, 0.01, 2000, thr=1e-7, patience=100)
+    assert info.stopped and info.steps_at_stop == 100 and info.steps_done == 101
 
 
 # ------------------------------------------- LDS-resident whole-run kernel
