@@ -15,8 +15,9 @@ constexpr int kWaves = kBlock / kWave;
 constexpr int kMaxN = 34;
 constexpr int kRateStride = 40;  // doubles per rate set (n+1 <= 35)
 
-// Device-resident loop state.  Written only by the last block of a step's
-// final kernel (and by the host between runs); read by every kernel at entry.
+// Device-resident loop state.  Written by the end-of-step reduction (one
+// thread of k_finish_step / k_batch_finish, or of the LDS-resident kernel)
+// and by the host between runs; read by every kernel at entry.
 struct RunState {
     double t;            // time after the last finished step
     double h;

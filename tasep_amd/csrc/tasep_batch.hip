@@ -148,12 +148,24 @@ int build_tables(tasep_batch *b, int sched) {
     std::vector<size_t> order(bt.size());
     for (size_t j = 0; j < order.size(); j++) order[j] = j;
     std::stable_sort(order.begin(), order.end(), [&](size_t p, size_t q) { return bt[p].nblocks > bt[q].nblocks; });
+    // An instance larger than an eighth of all blocks is split into eight
+    // contiguous runs, one per XCD (a lone large lattice still fills the chip).
+    uint64_t total_blocks = 0;
+    for (const BatchInst &bi : bt) total_blocks += bi.nblocks;
     std::vector<std::vector<BatchBlock>> lane(8);
     for (size_t j : order) {
+        const uint32_t nb = bt[j].nblocks;
+        if (nb >= 8 && (uint64_t)nb * 8 > total_blocks) {
+            for (uint32_t x = 0; x < 8; x++)
+                for (uint32_t k = x * (nb / 8); k < (x + 1) * (nb / 8); k++)
+                    lane[x].push_back(BatchBlock{(uint32_t)j, k});
+            for (uint32_t k = 8 * (nb / 8); k < nb; k++) lane[k % 8].push_back(BatchBlock{(uint32_t)j, k});
+            continue;
+        }
         size_t best = 0;
         for (size_t x = 1; x < 8; x++)
             if (lane[x].size() < lane[best].size()) best = x;
-        for (uint32_t k = 0; k < bt[j].nblocks; k++) lane[best].push_back(BatchBlock{(uint32_t)j, k});
+        for (uint32_t k = 0; k < nb; k++) lane[best].push_back(BatchBlock{(uint32_t)j, k});
     }
     size_t longest = 0;
     for (auto &l : lane) longest = std::max(longest, l.size());

@@ -3,8 +3,9 @@
 // The context owns the state vector and work vectors in HBM, a run-state
 // block, per-block partial buffers and one default-priority stream.  A run
 // enqueues the step kernels in chunks; the early-exit decision is made on
-// the device by the last block of each step (finish_step), so the host only
-// reads one flag per chunk.
+// the device by the end-of-step launch (k_finish_step), so the host only
+// reads one flag per chunk.  Lattices with n <= 12 run whole inside one
+// workgroup's LDS (k_lds_run), one launch per up to 65536 steps.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
