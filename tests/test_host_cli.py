@@ -123,3 +123,31 @@ def test_host_snapshots_option(tmp_path):
     assert len(prb) == 10 and len(prb[0].split(", ")) == 1 + 32
     assert os.path.isdir(tmp_path / "out_covariance_data")
     assert os.path.exists(tmp_path / "out_highcorr_timed")
+
+
+@needs_bin
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,fast", [(14, False), (20, False), (20, True)])
+def test_host_large_lattice_vs_oracle(tmp_path, n, fast):
+    """Larger lattices than the fixtures (other kernels: one lane per state,
+    tiles, CSR): the probabilities file equals the oracle's reference loop
+    bit for bit, and the high-order files equal those of the oracle's
+    window marginals."""
+    import oracle
+    from tests import _highorder as H
+    rng = np.random.default_rng(n)
+    r = rng.uniform(0.2, 1.5, n + 1)
+    (tmp_path / "rates.txt").write_text(" ".join("%.17g" % v for v in r))
+    steps, T = 30, 0.3
+    args = ["-n", str(n), "-i", "rates.txt", "-t", "0", "-T", str(T), "-s", str(steps), "-o", "out"]
+    if fast:
+        args.append("-f")
+    p = _run(args, tmp_path)
+    assert p.returncode == 0, p.stderr
+    h = oracle.step_h(T, 0.0, steps)
+    ox, _ = oracle.run_ref(n, r, oracle.uniform_x0(n), 0.0, h, steps, use_csr=fast)
+    vals = np.array([float(ln.split("\t\t")[1]) for ln in open(tmp_path / "out")], np.float64)
+    assert np.array_equal(vals.view(np.uint64), ox.view(np.uint64))
+    W = oracle.windows(n, ox)
+    for fname, text in H.files(W, n).items():
+        assert open(tmp_path / fname).read() == text, fname
