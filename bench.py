@@ -203,6 +203,7 @@ def config_1(T, ev):
         with T.Engine(n, "f64", "ref", gen) as e:
             e.set_option(T.OPT_LDS, lds)
             e.set_rates(r)
+            e.run(0.0, h, 20)                  # warm-up: code objects, CSR build
             e.set_x0_state(0)
             info, ms = timed_run(ev, e, 0.0, h, 10000)
             xs[name] = e.get_x()
@@ -227,6 +228,7 @@ def config_2(T, ev):
                                   ("rk4_mf", "rk4", "mf", 1e-7), ("rk4_mf_thr1e-10", "rk4", "mf", 1e-10)):
         with T.Engine(n, "f64", integ, gen) as e:
             e.set_rates(r)
+            e.run(0.0, h, 20)                  # warm-up: code objects, CSR build
             e.set_x0_uniform()
             info, ms = timed_run(ev, e, 0.0, h, 20000, thr=thr, patience=100)
             if name.startswith("ref"):

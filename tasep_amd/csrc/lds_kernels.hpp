@@ -38,6 +38,9 @@ struct LdsArgs {
     const double *tv_amp, *tv_period, *tv_offset, *tv_c;
     RunState *st;      // run states, indexed by LdsInstance::id
     uint64_t steps;    // steps of this launch (fewer if an instance stops)
+    // INTEG 2 (reference loop through the stored matrix, one instance)
+    const uint64_t *rowPtr, *colInd;
+    const double *val;
 };
 
 // r(t) = max(0, A f(w t + phi) + c), f = sin / cos / linear
@@ -158,7 +161,9 @@ __device__ __forceinline__ A rk_row_lds(int n, uint32_t s, const G &u, const Lds
 // (stored stages).
 __host__ __device__ constexpr int lds_buffers(int integ) { return integ == 0 ? 2 : 4; }
 
-// INTEG: TASEP_REF semantics (0), RK4 (1).  Lane t handles states
+// INTEG: TASEP_REF semantics (0), RK4 (1), TASEP_REF through the stored CSR
+// (2: T = sum of val * x over the row in CSR order, src/lib/solv.c:8-13;
+// the arrays stay in global memory, x in LDS).  Lane t handles states
 // s = t, t + blockDim.x, ... < 2^n; everything an instance needs lives in
 // LDS, so registers hold only the row in flight and any 2^n fits the same
 // code.  TV = false: constant rates (RK4 in Horner form).  TV = true: rates
@@ -218,7 +223,7 @@ __global__ __launch_bounds__(1024) void k_lds_run(const LdsArgs a) {
     __syncthreads();
     LdsRates<double> qd;
     LdsRates<V> qv;
-    if (INTEG == 0) load_rates(qd, rr[0][0], n);
+    if (INTEG != 1) load_rates(qd, rr[0][0], n);
     else load_rates(qv, rv[0][0], n);
     if (INTEG == 1 && !TV) {  // Horner form: the diagonal once, in bC
         for (uint32_t s = t; s < D; s += nt) bC[s] = rk_diag_lds<V>(n, s, qv);
@@ -233,13 +238,21 @@ __global__ __launch_bounds__(1024) void k_lds_run(const LdsArgs a) {
     };
     for (uint64_t k = 0; k < a.steps && !stopped; k++) {
         Kahan ck;
-        if constexpr (INTEG == 0) {
+        if constexpr (INTEG == 0 || INTEG == 2) {
             // reference loop: x += (((k+2k)+2k)+k)/6 with k = h (L x)_s
             if (TV) load_rates(qd, rr[par][0], n);
             auto gx = [&](uint32_t j) { return cur[j]; };
 #pragma unroll 2
             for (uint32_t s = t; s < D; s += nt) {
-                const double d = ref_delta(h, ref_row_lds(n, s, gx, qd, ref_diag_lds(n, s, qd)));
+                double T;
+                if constexpr (INTEG == 2) {
+                    T = 0.0;
+                    const uint64_t e = a.rowPtr[s + 1];
+                    for (uint64_t j = a.rowPtr[s]; j < e; j++) T += a.val[j] * (double)cur[a.colInd[j]];
+                } else {
+                    T = ref_row_lds(n, s, gx, qd, ref_diag_lds(n, s, qd));
+                }
+                const double d = ref_delta(h, T);
                 nxt[s] = cur[s] + d;
                 ck.add(fabs(d));
             }

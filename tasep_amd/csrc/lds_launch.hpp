@@ -51,11 +51,12 @@ inline void launch_lds_k(const LdsArgs &a, int blocks, int nt, size_t shmem, hip
 
 // One launch of k_lds_run over `blocks` instances whose largest needs
 // `shmem` bytes of LDS.  tv: some instance has time-variant rates, or RK4
-// must keep stored stages.
+// must keep stored stages; csr: the reference loop reads the stored matrix.
 inline void launch_lds_any(int prec, int integ, bool tv, const LdsArgs &a, int blocks, int nt, size_t shmem,
-                           hipStream_t s) {
+                           hipStream_t s, bool csr = false) {
     if (integ == TASEP_REF) {
-        if (tv) launch_lds_k<double, 0, true>(a, blocks, nt, shmem, s);
+        if (csr) launch_lds_k<double, 2, false>(a, blocks, nt, shmem, s);
+        else if (tv) launch_lds_k<double, 0, true>(a, blocks, nt, shmem, s);
         else launch_lds_k<double, 0, false>(a, blocks, nt, shmem, s);
     } else if (prec == TASEP_F64) {
         if (tv) launch_lds_k<double, 1, true>(a, blocks, nt, shmem, s);

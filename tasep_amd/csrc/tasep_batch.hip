@@ -569,6 +569,8 @@ int tasep_batch_run(tasep_batch *b, double t0, double h, uint64_t steps, double 
         a.tv_offset = b->tv_par + 2 * stride;
         a.tv_c = b->tv_par + 3 * stride;
         a.st = b->st;
+        a.rowPtr = a.colInd = nullptr;
+        a.val = nullptr;
         const int groups[2][2] = {{0, b->lds_const}, {b->lds_const, (int)b->lds_ids.size()}};
         for (int gi = 0; gi < 2; gi++) {
             const int lo = groups[gi][0], hi = groups[gi][1];

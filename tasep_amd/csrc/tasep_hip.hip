@@ -620,7 +620,8 @@ static int lds_threads(const tasep_ctx *c) {
 constexpr int kLdsAutoMaxN = 12;
 
 static bool use_lds(const tasep_ctx *c) {
-    if (c->gen != TASEP_MF || !c->lds || !lds_fits(c->n, (int)c->esize, c->integ)) return false;
+    const bool csr_ref = c->gen == TASEP_CSR && c->integ == TASEP_REF;
+    if ((c->gen != TASEP_MF && !csr_ref) || !c->lds || !lds_fits(c->n, (int)c->esize, c->integ)) return false;
     return c->lds == 2 || c->n <= kLdsAutoMaxN;
 }
 
@@ -765,12 +766,17 @@ int tasep_run_resume(tasep_ctx *c, double t0, double h, uint64_t steps, double t
         a.tv_offset = c->tv_par ? c->tv_par + 2 * kRateStride : nullptr;
         a.tv_c = c->tv_par ? c->tv_par + 3 * kRateStride : nullptr;
         a.st = c->st;
+        a.rowPtr = c->rowPtr;
+        a.colInd = c->colInd;
+        a.val = c->val;
+        const bool csr = c->gen == TASEP_CSR;
         const bool tv = c->tv || (c->integ == TASEP_RK4 && c->force_staged);
         uint64_t chunk = ((uint64_t)1 << 27) >> c->n;
         if (chunk > 65536) chunk = 65536;
         for (uint64_t j = 0; j < steps; j += chunk) {
             a.steps = steps - j < chunk ? steps - j : chunk;
-            launch_lds_any(c->prec, c->integ, tv, a, 1, nt, lds_bytes(c->n, (int)c->esize, c->integ), c->stream);
+            launch_lds_any(c->prec, c->integ, tv, a, 1, nt, lds_bytes(c->n, (int)c->esize, c->integ), c->stream,
+                           csr);
         }
         HIP_OK(hipGetLastError());
         return finish_run(c, info, true);

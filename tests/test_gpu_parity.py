@@ -462,6 +462,25 @@ def test_lds_ref_early_stop_and_chunks_match_streaming():
     assert iu.stopped and iu.steps_at_stop == 100 and iu.steps_done == 101
 
 
+def test_lds_csr_reference_loop_matches_streaming():
+    """The stored-CSR reference loop inside one workgroup: bitwise equal to
+    the one-launch-per-step CSR kernel and to the matrix-free runs."""
+    n = 11
+    r = np.random.default_rng(33).uniform(0.3, 1.5, n + 1)
+    h = oracle.step_h(50.0, 0.0, 5000)
+    out = []
+    for gen, lds in (("csr", 1), ("csr", 0), ("mf", 1)):
+        with T.Engine(n, "f64", "ref", gen) as e:
+            e.set_option(T.OPT_LDS, lds)
+            e.set_rates(r)
+            e.set_x0_state(0)
+            info = e.run(0.0, h, 5000, thr=1e-7, patience=100)
+            out.append((e.get_x(), info))
+    for x, info in out[1:]:
+        assert _bits_equal(x, out[0][0])
+        assert info.steps_done == out[0][1].steps_done and info.t == out[0][1].t
+
+
 def test_lds_converged_exit_matches_streaming():
     n = 8
     r = np.random.default_rng(31).uniform(0.3, 1.5, n + 1)
