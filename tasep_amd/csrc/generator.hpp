@@ -65,17 +65,6 @@ __device__ __forceinline__ double ref_row_mf(const Lattice g, uint64_t s,
     return T;
 }
 
-__device__ __forceinline__ double ref_row_csr(uint64_t s,
-                                              const uint64_t *__restrict__ rowPtr,
-                                              const uint64_t *__restrict__ colInd,
-                                              const double *__restrict__ val,
-                                              const double *__restrict__ x) {
-    double T = 0.0;
-    const uint64_t e = rowPtr[s + 1];
-    for (uint64_t j = rowPtr[s]; j < e; j++) T += val[j] * x[colInd[j]];
-    return T;
-}
-
 // The reference update for one state: all four stages equal k = h*T
 // (src/lib/solv.c:12-13 reads the unmodified x), combined with the
 // reference's rounding (src/masterEquation.c:640).

@@ -27,7 +27,6 @@ struct RefArgs {
     Lattice g;
 };
 
-template <bool CSR>
 __global__ __launch_bounds__(kBlock) void k_ref_step(const RefArgs a) {
     if (a.st->stopped) return;
     __shared__ double r[kRateStride];
@@ -36,8 +35,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_step(const RefArgs a) {
     Kahan chk, mass;
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     for (uint64_t s = (uint64_t)blockIdx.x * kBlock + threadIdx.x; s < a.dim; s += stride) {
-        double T = CSR ? ref_row_csr(s, a.rowPtr, a.colInd, a.val, a.x)
-                       : ref_row_mf(a.g, s, a.x, r);
+        double T = ref_row_mf(a.g, s, a.x, r);
         double d = ref_delta(a.h, T);
         double xn = a.x[s] + d;
         a.xn[s] = xn;
@@ -46,6 +44,54 @@ __global__ __launch_bounds__(kBlock) void k_ref_step(const RefArgs a) {
     }
     double bc = block_sum(chk.value(), red);
     double bm = block_sum(mass.value(), red);
+    store_partials(a.pchk, a.pmass, bc, bm);
+}
+
+// The reference loop through the stored CSR with coalesced matrix reads.
+// One wave takes 64 consecutive rows; their entries are one contiguous
+// segment of colInd/val, which the wave copies to LDS in chunks of 512
+// (every lane loading 8 consecutive entries), then each lane sums its own
+// row from LDS in CSR order (src/lib/solv.c:8-13) -- where lanes that each
+// walked their own row in global memory touched 64 different cache lines
+// per load.  A row that crosses a chunk keeps its partial sum.
+constexpr int kCsrChunk = 512;
+
+__global__ __launch_bounds__(kWave) void k_ref_csr_wave(const RefArgs a) {
+    if (a.st->stopped) return;
+    __shared__ uint64_t col[kCsrChunk];
+    __shared__ double val[kCsrChunk];
+    __shared__ double red[1];
+    const int t = threadIdx.x;
+    Kahan chk, mass;
+    const uint64_t groups = (a.dim + kWave - 1) / kWave;
+    for (uint64_t gi = blockIdx.x; gi < groups; gi += gridDim.x) {
+        const uint64_t s0 = gi * kWave, s = s0 + t;
+        const uint64_t last = s0 + kWave < a.dim ? s0 + kWave : a.dim;
+        const uint64_t seg0 = a.rowPtr[s0], seg1 = a.rowPtr[last];
+        uint64_t j = s < a.dim ? a.rowPtr[s] : seg1;
+        const uint64_t je = s < a.dim ? a.rowPtr[s + 1] : seg1;
+        double T = 0.0;
+        for (uint64_t base = seg0; base < seg1; base += kCsrChunk) {
+            const uint64_t cend = base + kCsrChunk < seg1 ? base + kCsrChunk : seg1;
+            __syncthreads();  // the previous chunk is consumed
+            for (uint64_t e = base + t; e < cend; e += kWave) {
+                col[e - base] = a.colInd[e];
+                val[e - base] = a.val[e];
+            }
+            __syncthreads();
+            for (; j < je && j < cend; j++) T += val[j - base] * a.x[col[j - base]];
+        }
+        if (s < a.dim) {
+            const double d = ref_delta(a.h, T);
+            const double xn = a.x[s] + d;
+            a.xn[s] = xn;
+            chk.add(fabs(d));
+            mass.add(xn);
+        }
+    }
+    const double bc = wave_sum(chk.value());
+    const double bm = wave_sum(mass.value());
+    (void)red;
     store_partials(a.pchk, a.pmass, bc, bm);
 }
 

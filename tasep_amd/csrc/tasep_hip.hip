@@ -602,10 +602,14 @@ static void ref_step(tasep_ctx *c, double h, uint64_t j) {
     a.g = make_lattice(c->n);
     prof_begin(c);
     if (c->gen == TASEP_CSR) {
-        hipLaunchKernelGGL(k_ref_step<true>, dim3(grid_for(c->dim)), dim3(kBlock), 0, c->stream, a);
+        const uint64_t groups = (c->dim + kWave - 1) / kWave;
+        const uint32_t g = (uint32_t)(groups < (uint64_t)kMaxBlocks ? groups : kMaxBlocks);
+        hipLaunchKernelGGL(k_ref_csr_wave, dim3(g), dim3(kWave), 0, c->stream, a);
         prof_end(c, "ref_step_csr", 24.0 + 16.0 * (2.0 + (c->n - 1) / 4.0));
+        launch_finish(c, g);
+        return;
     } else {
-        hipLaunchKernelGGL(k_ref_step<false>, dim3(grid_for(c->dim)), dim3(kBlock), 0, c->stream, a);
+        hipLaunchKernelGGL(k_ref_step, dim3(grid_for(c->dim)), dim3(kBlock), 0, c->stream, a);
         prof_end(c, "ref_step_mf", 16.0);
     }
     launch_finish(c, (uint32_t)grid_for(c->dim));
