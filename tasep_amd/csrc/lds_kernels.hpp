@@ -242,7 +242,6 @@ __global__ __launch_bounds__(1024) void k_lds_run(const LdsArgs a) {
             // reference loop: x += (((k+2k)+2k)+k)/6 with k = h (L x)_s
             if (TV) load_rates(qd, rr[par][0], n);
             auto gx = [&](uint32_t j) { return cur[j]; };
-#pragma unroll 2
             for (uint32_t s = t; s < D; s += nt) {
                 double T;
                 if constexpr (INTEG == 2) {
@@ -268,14 +267,12 @@ __global__ __launch_bounds__(1024) void k_lds_run(const LdsArgs a) {
                 const V *src = p == 0 ? xs : (p == 1 ? bA : bB);
                 V *dst = p == 1 ? bB : bA;
                 auto gp = [&](uint32_t j) { return src[j]; };
-#pragma unroll 2
                 for (uint32_t s = t; s < D; s += nt)
                     dst[s] = fma(cs[p], rk_row_lds<V>(n, s, gp, qv, bC[s]), xs[s]);
                 __syncthreads();
             }
             auto gp = [&](uint32_t j) { return bA[j]; };
             const V hv = (V)h;
-#pragma unroll 2
             for (uint32_t s = t; s < D; s += nt) {
                 const V d = hv * rk_row_lds<V>(n, s, gp, qv, bC[s]);
                 xs[s] = xs[s] + d;  // only lane-own entries of xs are touched here
@@ -294,14 +291,12 @@ __global__ __launch_bounds__(1024) void k_lds_run(const LdsArgs a) {
                 auto gp = [&](uint32_t j) { return src[j]; };
                 if (p != 2) load_rates(qv, rv[par][p == 0 ? 0 : (p == 1 ? 1 : 2)], n);
                 if (p < 3) {
-#pragma unroll 2
                     for (uint32_t s = t; s < D; s += nt) {
                         const V kk = hv * rk_row_lds<V>(n, s, gp, qv, rk_diag_lds<V>(n, s, qv));
                         bC[s] = p == 0 ? kk : bC[s] + (V)2 * kk;
                         dst[s] = fma(cy[p], kk, xs[s]);
                     }
                 } else {
-#pragma unroll 2
                     for (uint32_t s = t; s < D; s += nt) {
                         const V kk = hv * rk_row_lds<V>(n, s, gp, qv, rk_diag_lds<V>(n, s, qv));
                         const V d = (bC[s] + kk) / (V)6;
