@@ -31,8 +31,8 @@ struct RunState {
     int32_t patience;    // < 0: never leave early
     int32_t stopped;
     int32_t renorm;      // 1: scale <- 1/mass after each step
-    uint32_t counter;    // blocks of the current kernel that have finished
-    uint32_t pad;
+    uint32_t pad0, pad1;
+    double t_rates;      // start time of the next step whose rates k_tv_rates evaluates
 };
 
 // Neumaier-compensated FP64 accumulator.
@@ -69,10 +69,6 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
     return r;
 }
 
-__device__ __forceinline__ double load_relaxed(const double *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Advance a run state by one finished step with its checksum sum|dx| and
 // mass sum x, exactly as the reference loop does
 // (src/masterEquation.c:644,670-679): t += h, then the sub-threshold counter
@@ -97,14 +93,16 @@ __device__ __forceinline__ void advance_state(RunState *st, double chk, double m
 
 // Fixed-order FP64 sums of nslots block partials (every thread of the
 // block gets both), so the result does not depend on block scheduling.
+// The partials come from an earlier launch, so plain loads see them.
 template <int NT = kBlock>
-__device__ __forceinline__ void reduce_partials(const double *part_chk, const double *part_mass, uint32_t nslots,
+__device__ __forceinline__ void reduce_partials(const double *__restrict__ part_chk,
+                                                const double *__restrict__ part_mass, uint32_t nslots,
                                                 double &chk, double &mass) {
     __shared__ double s_red[NT / kWave];
     Kahan kc, km;
     for (uint32_t b = threadIdx.x; b < nslots; b += NT) {
-        kc.add(load_relaxed(part_chk + b));
-        km.add(load_relaxed(part_mass + b));
+        kc.add(part_chk[b]);
+        km.add(part_mass[b]);
     }
     chk = block_sum<NT>(kc.value(), s_red);
     mass = block_sum<NT>(km.value(), s_red);

@@ -14,6 +14,7 @@
 namespace tasep_host {
 
 constexpr int kMaxBlocks = 8192;  // grid cap of the streaming kernels
+constexpr int kWindow = 128;      // steps per reduction window with the exit test armed
 constexpr int kMomMaxBlocks = 2048;
 
 // Message of the last error on this thread (tasep_strerror).

@@ -95,16 +95,40 @@ __global__ __launch_bounds__(kWave) void k_ref_csr_wave(const RefArgs a) {
     store_partials(a.pchk, a.pmass, bc, bm);
 }
 
+constexpr int kFinishThreads = 1024;
+
+// A window of m steps with the exit test armed (no exit is possible before
+// its last step): workgroup i sums step i's block partials (slot i of a
+// ring of `stride`-sized slots); k_window_apply then advances the run state
+// through the m steps in order, exactly as m end-of-step reductions would.
+__global__ __launch_bounds__(kFinishThreads) void k_window_sums(const double *__restrict__ pchk,
+                                                                const double *__restrict__ pmass,
+                                                                uint32_t nblocks, uint32_t stride,
+                                                                double *__restrict__ tot) {
+    double chk, mass;
+    reduce_partials<kFinishThreads>(pchk + (uint64_t)blockIdx.x * stride, pmass + (uint64_t)blockIdx.x * stride,
+                                    nblocks, chk, mass);
+    if (threadIdx.x == 0) {
+        tot[2 * blockIdx.x] = chk;
+        tot[2 * blockIdx.x + 1] = mass;
+    }
+}
+
+__global__ void k_window_apply(RunState *st, const double *__restrict__ tot, uint32_t m) {
+    if (threadIdx.x != 0) return;
+    for (uint32_t i = 0; i < m && !st->stopped; i++) advance_state(st, tot[2 * i], tot[2 * i + 1]);
+}
+
 // End of `steps` steps: sum the last final kernel's `nblocks` block
 // partials in a fixed order and advance the run state (t += h per step,
 // early-exit counter and test).  steps > 1 only for runs without an exit
 // test, whose intermediate checksums are not needed.
-__global__ __launch_bounds__(kBlock) void k_finish_step(RunState *st, const double *__restrict__ pchk,
+__global__ __launch_bounds__(kFinishThreads) void k_finish_step(RunState *st, const double *__restrict__ pchk,
                                                         const double *__restrict__ pmass, uint32_t nblocks,
                                                         uint32_t steps) {
     if (st->stopped) return;
     double chk, mass;
-    reduce_partials(pchk, pmass, nblocks, chk, mass);
+    reduce_partials<kFinishThreads>(pchk, pmass, nblocks, chk, mass);
     if (threadIdx.x == 0) {
         for (uint32_t i = 1; i < steps; i++) {
             st->t = st->t + st->h;
@@ -231,18 +255,23 @@ struct TvArgs {
 __global__ void k_tv_rates(RunState *st, TvArgs tv, double *rates, int n, int nsets) {
     if (st->stopped) return;
     const int i = threadIdx.x;
-    if (i > n) return;
-    const double t0 = st->t, h = st->h;
-    for (int set = 0; set < nsets; set++) {
-        const double t = set == 0 ? t0 : (set == 1 ? t0 + 0.5 * h : t0 + h);
-        double v;
-        switch (tv.type[i]) {
-        case 0: v = tv.amp[i] * sin(t * tv.period[i] + tv.offset[i]) + tv.c[i]; break;
-        case 1: v = tv.amp[i] * cos(t * tv.period[i] + tv.offset[i]) + tv.c[i]; break;
-        default: v = tv.amp[i] * t + tv.c[i]; break;
+    // the step's start time, on a clock of its own (t_rates += h per step,
+    // the same sums as t), so that t itself may be advanced later
+    const double t0 = st->t_rates, h = st->h;
+    if (i <= n) {
+        for (int set = 0; set < nsets; set++) {
+            const double t = set == 0 ? t0 : (set == 1 ? t0 + 0.5 * h : t0 + h);
+            double v;
+            switch (tv.type[i]) {
+            case 0: v = tv.amp[i] * sin(t * tv.period[i] + tv.offset[i]) + tv.c[i]; break;
+            case 1: v = tv.amp[i] * cos(t * tv.period[i] + tv.offset[i]) + tv.c[i]; break;
+            default: v = tv.amp[i] * t + tv.c[i]; break;
+            }
+            rates[set * kRateStride + i] = v < 0.0 ? 0.0 : v;
         }
-        rates[set * kRateStride + i] = v < 0.0 ? 0.0 : v;
     }
+    __syncthreads();  // every lane has read t_rates
+    if (i == 0) st->t_rates = t0 + h;
 }
 
 // ===========================================================================

@@ -53,9 +53,11 @@ struct tasep_ctx {
     int lds = 1;                // whole runs in one workgroup's LDS: 0 never, 1 n <= 12, 2 where it fits
     int lds_threads = 0;        // workgroup size of the LDS-resident kernel (0: automatic)
     uint64_t steps_before = 0;  // cumulative step count when the current run call began
-    bool finish_each = true;    // reduce every step (exit test or time-variant rates need it)
+    bool window = false;        // exit test armed: reductions run per window of steps
+    uint32_t slot = 0;          // partial-sum slot of the step being enqueued (window mode)
     uint32_t pending = 0;       // steps whose end-of-step reduction is deferred
     uint32_t pending_blocks = 0;
+    double *wtot = nullptr;     // per-step totals of a window (2 * kWindow doubles)
     LdsInstance *lds_inst = nullptr;       // device: the instance table of one entry
     LdsInstance *lds_inst_host = nullptr;  // pinned
 };
@@ -108,6 +110,7 @@ int tasep_ctx_destroy(tasep_ctx *c) {
     if (c->st_host) hipHostFree(c->st_host);
     hipFree(c->pchk);
     hipFree(c->pmass);
+    hipFree(c->wtot);
     hipFree(c->mom_part);
     hipFree(c->mom_out);
     hipFree(c->lds_inst);
@@ -157,8 +160,9 @@ int tasep_ctx_create(tasep_ctx **out, int n, int precision, int integrator, int 
     CK(hipck(hipMalloc(&c->st, sizeof(RunState)), "hipMalloc(state)"));
     CK(hipck(hipHostMalloc((void **)&c->st_host, sizeof(RunState), hipHostMallocDefault),
              "hipHostMalloc(state)"));
-    CK(hipck(hipMalloc(&c->pchk, kMaxBlocks * sizeof(double)), "hipMalloc(partials)"));
-    CK(hipck(hipMalloc(&c->pmass, kMaxBlocks * sizeof(double)), "hipMalloc(partials)"));
+    CK(hipck(hipMalloc(&c->pchk, (size_t)kWindow * kMaxBlocks * sizeof(double)), "hipMalloc(partials)"));
+    CK(hipck(hipMalloc(&c->pmass, (size_t)kWindow * kMaxBlocks * sizeof(double)), "hipMalloc(partials)"));
+    CK(hipck(hipMalloc(&c->wtot, (size_t)2 * kWindow * sizeof(double)), "hipMalloc(window)"));
     CK(hipck(hipMalloc(&c->lds_inst, sizeof(LdsInstance)), "hipMalloc(instance)"));
     CK(hipck(hipHostMalloc((void **)&c->lds_inst_host, sizeof(LdsInstance), hipHostMallocDefault),
              "hipHostMalloc(instance)"));
@@ -339,8 +343,8 @@ static PassArgs<V> pass_base(tasep_ctx *c) {
     a.colInd = c->colInd;
     a.val = c->val;
     a.st = c->st;
-    a.pchk = c->pchk;
-    a.pmass = c->pmass;
+    a.pchk = c->pchk + (size_t)c->slot * kMaxBlocks;
+    a.pmass = c->pmass + (size_t)c->slot * kMaxBlocks;
     a.dim = c->dim;
     a.g = make_lattice(c->n);
     return a;
@@ -368,24 +372,26 @@ static void prof_end(tasep_ctx *c, const char *name, double bytes_per_state) {
     }
 }
 
-// End-of-step reduction of the final kernel's nblocks partials.  Runs
-// without an exit test and with constant rates defer it to the end of the
-// run (one launch then accounts for all pending steps).
+// End-of-step reduction of the final kernel's nblocks partials.  With the
+// exit test armed the reductions run per window of steps (the run loop
+// launches them); runs without one defer the reduction to the end of the
+// run (one launch then accounts for all pending steps).  The per-launch
+// profile reduces every step.
 static void launch_finish(tasep_ctx *c, uint32_t nblocks) {
-    if (!c->finish_each && !g_prof) {
-        c->pending += 1;
+    if (!g_prof) {
         c->pending_blocks = nblocks;
+        if (!c->window) c->pending += 1;
         return;
     }
     prof_begin(c);
-    hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(kBlock), 0, c->stream, c->st, c->pchk, c->pmass, nblocks,
+    hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(kFinishThreads), 0, c->stream, c->st, c->pchk, c->pmass, nblocks,
                        1u);
     prof_end(c, "finish_step", 0.0);
 }
 
 static void flush_finish(tasep_ctx *c) {
     if (!c->pending) return;
-    hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(kBlock), 0, c->stream, c->st, c->pchk, c->pmass,
+    hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(kFinishThreads), 0, c->stream, c->st, c->pchk, c->pmass,
                        c->pending_blocks, c->pending);
     c->pending = 0;
 }
@@ -454,8 +460,8 @@ static TileArgs<V> tile_base(tasep_ctx *c) {
     memset(&a, 0, sizeof a);
     a.rates = c->rates;
     a.st = c->st;
-    a.pchk = c->pchk;
-    a.pmass = c->pmass;
+    a.pchk = c->pchk + (size_t)c->slot * kMaxBlocks;
+    a.pmass = c->pmass + (size_t)c->slot * kMaxBlocks;
     a.n = c->n;
     a.ntiles = c->dim >> TileGeom<V>::b;
     return a;
@@ -573,8 +579,8 @@ static void ref_step(tasep_ctx *c, double h, uint64_t j) {
         t.out = (double *)c->buf[c->cur ^ (int)((j + 1) & 1)];
         t.rates = c->rates;
         t.st = c->st;
-        t.pchk = c->pchk;
-        t.pmass = c->pmass;
+        t.pchk = c->pchk + (size_t)c->slot * kMaxBlocks;
+        t.pmass = c->pmass + (size_t)c->slot * kMaxBlocks;
         t.h = h;
         t.n = c->n;
         t.ntiles = c->dim >> TileGeom<double>::b;
@@ -595,8 +601,8 @@ static void ref_step(tasep_ctx *c, double h, uint64_t j) {
     a.colInd = c->colInd;
     a.val = c->val;
     a.st = c->st;
-    a.pchk = c->pchk;
-    a.pmass = c->pmass;
+    a.pchk = c->pchk + (size_t)c->slot * kMaxBlocks;
+    a.pmass = c->pmass + (size_t)c->slot * kMaxBlocks;
     a.dim = c->dim;
     a.h = h;
     a.g = make_lattice(c->n);
@@ -635,7 +641,8 @@ static int prepare_run(tasep_ctx *c, double t0, double h, double thr, int patien
     if (!c->have_rates) return fail(TASEP_ERR_STATE, "set rates before running");
     if (!isfinite(h) || !isfinite(t0)) return fail(TASEP_ERR_ARG, "t0 and h must be finite");
     c->pending = 0;
-    c->finish_each = c->tv || patience >= 0;
+    c->slot = 0;
+    c->window = !c->tv && patience >= 0;
     if (c->gen == TASEP_CSR && !c->csr_built) {
         int rc = tasep_build_csr(c);
         if (rc) return rc;
@@ -660,12 +667,13 @@ static int prepare_run(tasep_ctx *c, double t0, double h, double thr, int patien
         c->st_host->h = h;
         c->st_host->thr_h = thr * h;
         c->st_host->patience = c->tv ? -1 : patience;
-        c->st_host->counter = 0;
+        c->st_host->t_rates = c->st_host->t;
         HIP_OK(hipMemcpyAsync(c->st, c->st_host, sizeof(RunState), hipMemcpyHostToDevice, c->stream));
         return TASEP_OK;
     }
     memset(c->st_host, 0, sizeof(RunState));
     c->st_host->t = t0;
+    c->st_host->t_rates = t0;
     c->st_host->h = h;
     c->st_host->thr_h = thr * h;
     c->st_host->patience = c->tv ? -1 : patience;
@@ -785,12 +793,41 @@ int tasep_run_resume(tasep_ctx *c, double t0, double h, uint64_t steps, double t
         HIP_OK(hipGetLastError());
         return finish_run(c, info, true);
     }
-    const bool watch = c->st_host->patience >= 0;
-    // steps per host check of the exit flag: about 2^26 state-updates
+    const bool watch = c->window;
+    uint64_t j = 0;
+    if (watch) {
+        // Exit test armed: after e consecutive sub-threshold steps no exit is
+        // possible for the next patience + 1 - e steps, so those steps run
+        // back to back with their partials in ring slots, and one pair of
+        // launches reduces them and replays the exit logic step by step.
+        const int patience_ = c->st_host->patience;
+        int early = c->st_host->early_stop;
+        while (j < steps) {
+            uint64_t m = (uint64_t)(patience_ + 1 - early);
+            if (m < 1) m = 1;
+            if (m > (uint64_t)kWindow) m = kWindow;
+            if (m > steps - j) m = steps - j;
+            for (uint64_t k = 0; k < m; k++) {
+                c->slot = (uint32_t)k;
+                enqueue_step(c, h, j + k);
+            }
+            c->slot = 0;
+            hipLaunchKernelGGL(k_window_sums, dim3((uint32_t)m), dim3(kFinishThreads), 0, c->stream, c->pchk,
+                               c->pmass, c->pending_blocks, (uint32_t)kMaxBlocks, c->wtot);
+            hipLaunchKernelGGL(k_window_apply, dim3(1), dim3(64), 0, c->stream, c->st, c->wtot, (uint32_t)m);
+            HIP_OK(hipGetLastError());
+            j += m;
+            HIP_OK(hipMemcpyAsync(c->st_host, c->st, sizeof(RunState), hipMemcpyDeviceToHost, c->stream));
+            HIP_OK(hipStreamSynchronize(c->stream));
+            if (c->st_host->stopped) break;
+            early = c->st_host->early_stop;
+        }
+        return finish_run(c, info);
+    }
+    // steps per enqueued chunk: about 2^26 state-updates
     uint64_t chunk = ((uint64_t)1 << 26) >> c->n;
     if (chunk < 4) chunk = 4;
     if (chunk > 4096) chunk = 4096;
-    uint64_t j = 0;
     // Long runs replay a captured HIP graph of `chunk` steps (an even number,
     // so the ping-pong buffers keep their parity); the remainder and short
     // runs are launched directly.
@@ -807,37 +844,19 @@ int tasep_run_resume(tasep_ctx *c, double t0, double h, uint64_t steps, double t
         e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (e != hipSuccess) return fail(TASEP_ERR_HIP, "hipGraphInstantiate", e);
-        bool stopped = false;
-        while (j + chunk <= steps && !stopped) {
+        for (; j + chunk <= steps; j += chunk) {
             e = hipGraphLaunch(exec, c->stream);
             if (e != hipSuccess) {
                 hipGraphExecDestroy(exec);
                 return fail(TASEP_ERR_HIP, "hipGraphLaunch", e);
             }
             c->pending += per;
-            j += chunk;
-            if (watch && j < steps) {
-                HIP_OK(hipMemcpyAsync(&c->st_host->stopped, &c->st->stopped, sizeof(int32_t),
-                                      hipMemcpyDeviceToHost, c->stream));
-                HIP_OK(hipStreamSynchronize(c->stream));
-                stopped = c->st_host->stopped != 0;
-            }
         }
         HIP_OK(hipStreamSynchronize(c->stream));
         hipGraphExecDestroy(exec);
-        if (stopped) return finish_run(c, info);
     }
-    while (j < steps) {
-        const uint64_t end = j + chunk < steps ? j + chunk : steps;
-        for (; j < end; j++) enqueue_step(c, h, j);
-        HIP_OK(hipGetLastError());
-        if (watch && j < steps) {
-            HIP_OK(hipMemcpyAsync(&c->st_host->stopped, &c->st->stopped, sizeof(int32_t),
-                                  hipMemcpyDeviceToHost, c->stream));
-            HIP_OK(hipStreamSynchronize(c->stream));
-            if (c->st_host->stopped) break;
-        }
-    }
+    for (; j < steps; j++) enqueue_step(c, h, j);
+    HIP_OK(hipGetLastError());
     return finish_run(c, info);
 }
 

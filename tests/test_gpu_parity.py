@@ -481,6 +481,28 @@ def test_lds_csr_reference_loop_matches_streaming():
         assert info.steps_done == out[0][1].steps_done and info.t == out[0][1].t
 
 
+@pytest.mark.parametrize("gen", ["mf", "csr"])
+def test_windowed_exit_test_matches_oracle(gen):
+    """One lane per state with the exit test armed: steps run in windows in
+    which no exit is possible, then the reductions replay the exit logic
+    step by step; the stop step, t and x equal the oracle's reference loop."""
+    n = 13
+    r = np.random.default_rng(41).uniform(0.5, 1.5, n + 1)
+    h = 0.05
+    ox, oinfo = oracle.run_ref(n, r, oracle.uniform_x0(n), 0.0, h, 40000, thr=1e-7, patience=100,
+                               use_csr=gen == "csr")
+    assert oinfo.stopped
+    with T.Engine(n, "f64", "ref", gen) as e:
+        e.set_option(T.OPT_LDS, 0)
+        e.set_rates(r)
+        e.set_x0_uniform()
+        info = e.run(0.0, h, 40000, thr=1e-7, patience=100)
+        x = e.get_x()
+    assert info.stopped and info.steps_at_stop == oinfo.steps_at_stop
+    assert info.steps_done == oinfo.steps_done and info.t == oinfo.t
+    assert _bits_equal(x, ox)
+
+
 def test_lds_converged_exit_matches_streaming():
     n = 8
     r = np.random.default_rng(31).uniform(0.3, 1.5, n + 1)
