@@ -169,11 +169,47 @@ def cpu_baseline(n, rates, h, budget_s=20.0):
         if el > budget_s or (steps >= 3 and el > budget_s / 2):
             break
     rate = steps * dim / el
+    # the same loop on one thread, on a smaller lattice (a step at n=28 takes
+    # about a minute there)
+    n1 = min(n, 22)
+    d1 = 1 << n1
+    rp1, ci1, va1 = oracle.gencrs(n1, workload_rates(n1))
+    x1v = oracle.uniform_x0(n1)
+    k1 = [np.zeros(d1) for _ in range(4)]
+    xx1 = np.zeros(d1)
+    gomp.omp_set_num_threads(1)
+    try:
+        t1 = time.perf_counter()
+        s1 = 0
+        while True:
+            for i, k in enumerate(k1):
+                if use_ref:
+                    R.rk4_step(i + 1, d1, k, x1v, xx1, rp1, ci1, va1, h)
+                else:
+                    L.tor_apply_csr(d1, rp1, ci1, va1, x1v, h, k, 1)
+            L.tor_ref_update4(d1, x1v, k1[0], k1[1], k1[2], k1[3])
+            s1 += 1
+            e1 = time.perf_counter() - t1
+            if e1 > budget_s / 4 and s1 >= 2:
+                break
+    finally:
+        gomp.omp_set_num_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return dict(value=rate, unit="state-updates/s", cores=threads,
                 kind="reference" if use_ref else "port",
                 sample=(f"n={n}, {steps} steps of the -f CSR loop (4 row-parallel CSR "
                         f"SpMVs {'by the reference rk4_step built from source' if use_ref else 'restated'}"
-                        f" + serial update), FP64, {threads} OpenMP threads, {el:.1f} s"))
+                        f" + serial update), FP64, {threads} OpenMP threads, {el:.1f} s"),
+                one_thread={"value": s1 * d1 / e1, "n": n1, "steps": s1, "seconds": e1},
+                cpu_model=model)
 
 
 def step_h(T_end, t0, steps):
