@@ -47,7 +47,7 @@ struct tasep_ctx {
     // options (tasep_set_option)
     int tiled = 1;              // tiled matrix-free passes: 0 never, 1 n >= 18 (19 REF), 2 where they fit
     bool graphs = false;        // long runs replay a captured graph of steps
-    bool xcd_order = true;      // XCD-aware tile order
+    int xcd_order = 2;          // tile order of the tiled passes (tile_walk)
     bool force_staged = false;  // staged RK4 even for constant rates
     int tile_blocks = 0;        // persistent grid of the tiled passes (0: by occupancy)
     int lds = 1;                // whole runs in one workgroup's LDS: 0 never, 1 n <= 12, 2 where it fits
@@ -467,34 +467,56 @@ static TileArgs<V> tile_base(tasep_ctx *c) {
     return a;
 }
 
-// Grid of a tiled pass: 16 workgroups per resident slot (occupancy x
-// CUs), which balances the tail (measured at n=28: 0.92 vs 1.03 ms for one
+// Workgroups of a tiled kernel resident on the whole device at once
+// (occupancy x CUs), 0 if unknown.
+static int resident_slots(const void *kernel) {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTileThreads, 0) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        per_cu < 1 || cus < 1)
+        return 0;
+    return per_cu * cus;
+}
+
+// Grid and tile order of a tiled pass.  Grid: 16 workgroups per resident
+// slot, which balances the tail (measured at n=28: 0.92 vs 1.03 ms for one
 // resident wave), at most kMaxBlocks partials for the end-of-step
-// reduction.  Rounded down to a multiple of 8 XCDs.
-template <class V, bool TWO, int EPI, bool SELF>
-static int tile_grid_cap() {
-    static int cap = 0;
-    if (!cap) {
-        int per_cu = 0, dev = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_pass<V, TWO, EPI, SELF>,
-                                                         kTileThreads, 0) != hipSuccess ||
-            hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            per_cu < 1 || cus < 1)
-            return 1024;
-        cap = (per_cu * cus * 16) & ~7;
-        if (cap < 8) cap = 8;
-        if (cap > kMaxBlocks) cap = kMaxBlocks;
+// reduction, rounded down to a multiple of 8 XCDs.  Order (tile_walk): the
+// requested one where the grid allows it, else the next simpler one.
+struct TileLaunch {
+    uint64_t grid;
+    int order;
+    uint32_t res;
+};
+static TileLaunch tile_launch(const tasep_ctx *c, const void *kernel, int &slots_cache, uint64_t ntiles) {
+    if (!slots_cache) slots_cache = resident_slots(kernel);
+    const int slots = slots_cache;
+    uint64_t cap = slots > 0 ? ((uint64_t)slots * 16) & ~(uint64_t)7 : 1024;
+    if (cap < 8) cap = 8;
+    if (cap > (uint64_t)kMaxBlocks) cap = kMaxBlocks;
+    if (c->tile_blocks > 0) cap = (uint64_t)c->tile_blocks;
+    TileLaunch L;
+    L.grid = ntiles < cap ? ntiles : cap;
+    L.order = c->xcd_order;
+    if (L.order == 3) {
+        L.res = slots > 0 ? (uint32_t)slots : 1;
+        if ((L.grid % L.res) == 0 && (ntiles % L.grid) == 0) return L;
+        L.order = 2;
     }
-    return cap;
+    L.res = slots >= 8 ? (uint32_t)(slots / 8) : 1;
+    if (L.order == 2 && !((L.grid % 8) == 0 && ((L.grid / 8) % L.res) == 0 && (ntiles % L.grid) == 0)) L.order = 1;
+    if (L.order == 1 && !((ntiles % 8) == 0 && (L.grid % 8) == 0)) L.order = 0;
+    return L;
 }
 
 template <class V, bool TWO, int EPI, bool SELF = false>
 static void launch_tile(tasep_ctx *c, TileArgs<V> a, const char *name, double bytes) {
-    const uint64_t cap = c->tile_blocks > 0 ? (uint64_t)c->tile_blocks
-                                            : (uint64_t)tile_grid_cap<V, TWO, EPI, SELF>();
-    uint64_t g = a.ntiles < cap ? a.ntiles : cap;
-    a.xcd_order = (c->xcd_order && (a.ntiles % 8) == 0 && (g % 8) == 0) ? 1 : 0;
+    static int slots = 0;
+    const TileLaunch L = tile_launch(c, (const void *)k_tile_pass<V, TWO, EPI, SELF>, slots, a.ntiles);
+    const uint64_t g = L.grid;
+    a.xcd_order = L.order;
+    a.xcd_res = L.res;
     prof_begin(c);
     hipLaunchKernelGGL((k_tile_pass<V, TWO, EPI, SELF>), dim3(g), dim3(kTileThreads), 0, c->stream, a);
     prof_end(c, name, bytes);
@@ -551,22 +573,6 @@ static bool use_tile(const tasep_ctx *c) {
     return c->tiled == 2 || c->n >= kTileAutoMinN;
 }
 
-static int tile_ref_cap() {
-    static int cap = 0;
-    if (!cap) {
-        int per_cu = 0, dev = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_ref, kTileThreads, 0) != hipSuccess ||
-            hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            per_cu < 1 || cus < 1)
-            return 1024;
-        cap = (per_cu * cus * 16) & ~7;
-        if (cap < 8) cap = 8;
-        if (cap > kMaxBlocks) cap = kMaxBlocks;
-    }
-    return cap;
-}
-
 static bool use_tile_ref(const tasep_ctx *c) {
     if (!c->tiled || c->gen != TASEP_MF || c->n < TileGeom<double>::b + 1) return false;
     return c->tiled == 2 || c->n >= kTileRefAutoMinN;
@@ -584,9 +590,11 @@ static void ref_step(tasep_ctx *c, double h, uint64_t j) {
         t.h = h;
         t.n = c->n;
         t.ntiles = c->dim >> TileGeom<double>::b;
-        const uint64_t cap = c->tile_blocks > 0 ? (uint64_t)c->tile_blocks : (uint64_t)tile_ref_cap();
-        const uint64_t g = t.ntiles < cap ? t.ntiles : cap;
-        t.xcd_order = (c->xcd_order && (t.ntiles % 8) == 0 && (g % 8) == 0) ? 1 : 0;
+        static int slots = 0;
+        const TileLaunch L = tile_launch(c, (const void *)k_tile_ref, slots, t.ntiles);
+        const uint64_t g = L.grid;
+        t.xcd_order = L.order;
+        t.xcd_res = L.res;
         prof_begin(c);
         hipLaunchKernelGGL(k_tile_ref, dim3(g), dim3(kTileThreads), 0, c->stream, t);
         prof_end(c, "ref_tile", 16.0);
@@ -867,7 +875,10 @@ int tasep_set_option(tasep_ctx *c, int key, int64_t value) {
         if (value < 0 || value > 2) return fail(TASEP_ERR_ARG, "tiled: 0, 1 or 2");
         c->tiled = (int)value;
         break;
-    case TASEP_OPT_XCD_ORDER: c->xcd_order = value != 0; break;
+    case TASEP_OPT_XCD_ORDER:
+        if (value < 0 || value > 3) return fail(TASEP_ERR_ARG, "tile order: 0 to 3");
+        c->xcd_order = (int)value;
+        break;
     case TASEP_OPT_RK4_STAGED: c->force_staged = value != 0; break;
     case TASEP_OPT_GRAPH: c->graphs = value != 0; break;
     case TASEP_OPT_LDS:

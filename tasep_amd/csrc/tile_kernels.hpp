@@ -51,6 +51,15 @@ constexpr int kTileThreads = 512;
 #define TASEP_TILE_WAVES 2
 #endif
 constexpr int kTileWaves = kTileThreads / kWave;
+// Tile-bond streams issued per tile before its LDS phase (registers:
+// 16 per stream and lane, twice that in the staged passes); a tile with
+// more active tile bonds reads the rest two at a time afterwards.
+#ifndef TASEP_TILE_SLOTS
+#define TASEP_TILE_SLOTS 2
+#endif
+#ifndef TASEP_TILE_SLOTS_TWO
+#define TASEP_TILE_SLOTS_TWO 0
+#endif
 
 template <class V> struct TileGeom;
 // E states per lane per column (one 16-byte load), C = 4 columns per lane.
@@ -132,26 +141,46 @@ struct TileArgs {
     double *pchk, *pmass;
     int n;
     uint64_t ntiles;
-    int xcd_order;     // 1: XCD-aware tile order
+    int xcd_order;     // tile order (tile_walk)
+    uint32_t xcd_res;  // workgroups resident per XCD (order 2)
 };
 
 
-// Load E*C stage-input values of one lane: C columns 2^LB states apart,
-// starting at the wave-uniform base + lane offset lo (G = gx [+ ck*gk]).
-template <class V, bool TWO, int E, int C, int LB>
-__device__ __forceinline__ void load_cols(const TileArgs<V> &a, uint64_t base, uint32_t lo,
-                                          V (&g)[C][E]) {
+// Stage-input values of one lane as loaded: R columns 2^LB states apart of
+// gx and, for the staged RK4 passes, of gk.  Loading and combining are split
+// so that a load can be issued long before its value is used.
+template <class V, bool TWO, int R, int E>
+struct Cols {
+    V g[R][E];
+    V k[TWO ? R : 1][E];
+};
+
+template <class V, bool TWO, int E, int R, int LB>
+__device__ __forceinline__ void issue_cols(const TileArgs<V> &a, uint64_t base, uint32_t lo,
+                                           Cols<V, TWO, R, E> &s) {
 #pragma unroll
-    for (int c = 0; c < C; c++) ld16(a.gx + base + lo + ((uint32_t)c << LB), g[c]);
+    for (int c = 0; c < R; c++) ld16(a.gx + base + lo + ((uint32_t)c << LB), s.g[c]);
     if constexpr (TWO) {
 #pragma unroll
-        for (int c = 0; c < C; c++) {
-            V q[E];
-            ld16(a.gk + base + lo + ((uint32_t)c << LB), q);
-#pragma unroll
-            for (int j = 0; j < E; j++) g[c][j] = fma(a.ck, q[j], g[c][j]);
-        }
+        for (int c = 0; c < R; c++) ld16(a.gk + base + lo + ((uint32_t)c << LB), s.k[c]);
     }
+}
+
+// Column c, register j of the stage input G = gx [+ ck * gk].
+template <class V, bool TWO, int R, int E>
+__device__ __forceinline__ V col_value(const TileArgs<V> &a, const Cols<V, TWO, R, E> &s, int c, int j) {
+    if constexpr (TWO) return fma(a.ck, s.k[c][j], s.g[c][j]);
+    else return s.g[c][j];
+}
+
+// acc[c0 + c] += r * G[c] for the R loaded columns.
+template <class V, bool TWO, int E, int R, int C>
+__device__ __forceinline__ void fma_cols(const TileArgs<V> &a, V r, const Cols<V, TWO, R, E> &s,
+                                         V (&acc)[C][E], int c0 = 0) {
+#pragma unroll
+    for (int c = 0; c < R; c++)
+#pragma unroll
+        for (int j = 0; j < E; j++) acc[c0 + c][j] = fma(r, col_value(a, s, c, j), acc[c0 + c][j]);
 }
 
 template <class V, int E, int C, int LB>
@@ -160,17 +189,34 @@ __device__ __forceinline__ void load_cols_x(const V *p, uint64_t base, uint32_t 
     for (int c = 0; c < C; c++) ld_self<V, E>(p + base + lo + ((uint32_t)c << LB), g[c]);
 }
 
-template <class V, int E, int C>
-__device__ __forceinline__ void fma_cols(V r, const V (&nb)[C][E], V (&acc)[C][E]) {
-#pragma unroll
-    for (int c = 0; c < C; c++)
-#pragma unroll
-        for (int j = 0; j < E; j++) acc[c][j] = fma(r, nb[c][j], acc[c][j]);
-}
-
-__device__ __forceinline__ uint64_t tile_of(uint64_t it, uint64_t per_xcd, int xcd_order) {
-    // it = g + 8*slot for workgroup g's tiles when gridDim % 8 == 0
-    return xcd_order ? (it & 7u) * per_xcd + (it >> 3) : it;
+// Tiles of one workgroup: its k-th tile is base + k * step.
+//   order 0: grid order, T = g + k * G
+//   order 1: XCD eighths -- workgroup g runs on XCD g % 8 and walks that
+//            XCD's eighth of the tile range with stride G / 8
+//   order 2: XCD eighths, and the R workgroups resident on an XCD at a time
+//            (dispatched together) take R consecutive tiles per step, so the
+//            XCD sweeps its eighth in order and a tile-bond partner 2^j tiles
+//            ahead is read again while it still sits in that XCD's L2.
+//            Needs G % 8 == 0, (G / 8) % R == 0 and ntiles % G == 0.
+//   order 3: one sweep over the whole range: the R workgroups resident on
+//            the device take R consecutive tiles per step, so a partner
+//            2^j tiles ahead is read again 2^j tiles of traffic later (a
+//            window the memory-side cache holds for larger j than the
+//            per-XCD sweeps).  Needs G % R == 0 and ntiles % G == 0.
+struct TileWalk {
+    uint64_t base, step;
+};
+__device__ __forceinline__ TileWalk tile_walk(uint64_t ntiles, int order, uint32_t res) {
+    const uint64_t g = blockIdx.x, G = gridDim.x;
+    if (order == 0) return {g, G};
+    if (order == 3) {
+        const uint64_t K = ntiles / G;                             // tiles per workgroup
+        return {g % res + (uint64_t)res * K * (g / res), res};
+    }
+    const uint64_t per_xcd = ntiles >> 3, q = g >> 3;
+    if (order == 1) return {(g & 7) * per_xcd + q, G >> 3};
+    const uint64_t K = per_xcd / (G >> 3);                         // tiles per workgroup
+    return {(g & 7) * per_xcd + q % res + (uint64_t)res * K * (q / res), res};
 }
 
 // Pop the lowest set bit of a uniform bond mask: returns the bond index
@@ -232,31 +278,59 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
     const V d_c0 = lane_hi ? (V)0 : r_c0;                          // out-flow for columns 1, 3
     const bool use_self = (EPI != EPI_STORE) || (!SELF && a.use_self);
     Kahan chk, mass;
-    const uint64_t per_xcd = a.ntiles >> 3;
+    const TileWalk walk = tile_walk(a.ntiles, a.xcd_order, a.xcd_res);
     const uint64_t stride = gridDim.x;
-    const uint64_t last = a.ntiles - 1;
 
-    uint64_t it = blockIdx.x;
-    V g[C][E];
+    uint64_t it = blockIdx.x, kt = 0;
+    constexpr int NS = TWO ? TASEP_TILE_SLOTS_TWO : TASEP_TILE_SLOTS;
+    using Str = Cols<V, TWO, C, E>;
+    Str cur, gn;                                                   // own vectors: this tile, the next
     if (it < a.ntiles)  // prologue: own vector of the first tile
-        load_cols<V, TWO, E, C, LB>(a, (tile_of(it, per_xcd, a.xcd_order) << TB0) + wofs, lo, g);
-    for (; it < a.ntiles; it += stride) {
-        const uint64_t T = tile_of(it, per_xcd, a.xcd_order);
+        issue_cols<V, TWO, E, C, LB>(a, (walk.base << TB0) + wofs, lo, cur);
+    for (; it < a.ntiles; it += stride, kt++) {
+        const uint64_t T = walk.base + kt * walk.step;
         const uint64_t wb = (T << TB0) + wofs;                     // wave base (uniform)
         const bool t_lo = T & 1, t_hi = (T >> (tb - 1)) & 1;
-        // ---- issue: first tile-bond stream, self vector, next tile's own vector
+        // own vector: combined here in the staged passes (frees the raw
+        // registers), at the LDS store otherwise (the first tile's streams
+        // are issued before its own vector has arrived)
+        V g[C][E];
+        if constexpr (TWO) {
+#pragma unroll
+            for (int c = 0; c < C; c++)
+#pragma unroll
+                for (int j = 0; j < E; j++) g[c][j] = col_value(a, cur, c, j);
+        }
+        // ---- issue every global read of this tile in the order it is used:
+        // bond LB+1 (columns 2, 3 read columns 0, 1 of tile T+1), bond n-4
+        // (per wave), the first NS tile bonds, the self vector, then the next
+        // tile's own vector; all are in flight during the LDS phase
+        Cols<V, TWO, 2, E> seam;
+        if (!t_lo) issue_cols<V, TWO, E, 2, LB>(a, wb + ((uint64_t)1 << TB0), lo, seam);
+        const bool has_n4 = t_hi && !w0;
+        Str s4;
+        if (!TWO && has_n4) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << (n - 4)), lo, s4);
         uint64_t mm = T & ~(T >> 1) & tmask;
-        int kA = pop_bond(mm, TB0);
-        V A[C][E], B[C][E];
-        load_cols<V, TWO, E, C, LB>(a, wb + (kA >= 0 ? ((uint64_t)1 << kA) : 0), lo, A);
+        int ks[NS > 0 ? NS : 1] = {-1};
+        Str S[NS > 1 ? NS : 2];
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
+            ks[q] = pop_bond(mm, TB0);
+            if (ks[q] >= 0) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << ks[q]), lo, S[q]);
+        }
         V xs[C][E];
         if constexpr (EPI != EPI_STORE || !SELF) load_cols_x<V, E, C, LB>(a.x, wb, lo, xs);
-        V gn[C][E];
-        {
-            const uint64_t itn = it + stride <= last ? it + stride : it;
-            load_cols<V, TWO, E, C, LB>(a, (tile_of(itn, per_xcd, a.xcd_order) << TB0) + wofs, lo, gn);
-        }
+        // (the last tile reloads itself in the staged passes, which keeps
+        // cur = gn a plain register renaming there)
+        const bool more = it + stride < a.ntiles;
+        if (more || TWO) issue_cols<V, TWO, E, C, LB>(a, ((more ? T + walk.step : T) << TB0) + wofs, lo, gn);
         // ---- own vector to LDS
+        if constexpr (!TWO) {
+#pragma unroll
+            for (int c = 0; c < C; c++)
+#pragma unroll
+                for (int j = 0; j < E; j++) g[c][j] = col_value(a, cur, c, j);
+        }
         __syncthreads();  // the previous tile's LDS reads are done
 #pragma unroll
         for (int c = 0; c < C; c++) st16(&tile[lds0 + c * LCOL], g[c]);
@@ -338,41 +412,30 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
         for (int c = 0; c < C; c++)
 #pragma unroll
             for (int j = 0; j < E; j += 2) acc[c][j] = fma(beta, g[c][j + 1], acc[c][j]);
-        // ---- global streams: bond LB+1 (columns 2, 3 from tile T+1), bond
-        // n-4 (per wave), then the tile bonds, two in flight at a time
-        if (!t_lo) {
-            V nb[2][E];
-            ld16(a.gx + wb + lo + ((uint64_t)1 << TB0), nb[0]);
-            ld16(a.gx + wb + lo + ((uint64_t)1 << TB0) + (1u << LB), nb[1]);
-            if constexpr (TWO) {
-                V q0[E], q1[E];
-                ld16(a.gk + wb + lo + ((uint64_t)1 << TB0), q0);
-                ld16(a.gk + wb + lo + ((uint64_t)1 << TB0) + (1u << LB), q1);
-#pragma unroll
-                for (int j = 0; j < E; j++) {
-                    nb[0][j] = fma(a.ck, q0[j], nb[0][j]);
-                    nb[1][j] = fma(a.ck, q1[j], nb[1][j]);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < E; j++) {
-                acc[2][j] = fma(r_c2, nb[0][j], acc[2][j]);
-                acc[3][j] = fma(r_c2, nb[1][j], acc[3][j]);
-            }
+        // ---- global streams, in issue order; tile bonds beyond the first NS
+        // two at a time
+        if (!t_lo) fma_cols<V, TWO, E, 2, C>(a, r_c2, seam, acc, 2);
+        if (has_n4) {
+            // the staged passes (two vectors per stream) issue it here: at
+            // 256 registers they cannot hold it through the LDS phase
+            if constexpr (TWO) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << (n - 4)), lo, s4);
+            fma_cols<V, TWO, E, C, C>(a, r_n4, s4, acc);
         }
-        if (t_hi && !w0) {
-            V nb[C][E];
-            load_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << (n - 4)), lo, nb);
-            fma_cols<V, E, C>(r_n4, nb, acc);
-        }
-        while (kA >= 0) {
-            const int kB = pop_bond(mm, TB0);
-            load_cols<V, TWO, E, C, LB>(a, wb + (kB >= 0 ? ((uint64_t)1 << kB) : 0), lo, B);
-            fma_cols<V, E, C>(rr[n - 1 - kA], A, acc);
-            if (kB < 0) break;
-            kA = pop_bond(mm, TB0);
-            load_cols<V, TWO, E, C, LB>(a, wb + (kA >= 0 ? ((uint64_t)1 << kA) : 0), lo, A);
-            fma_cols<V, E, C>(rr[n - 1 - kB], B, acc);
+#pragma unroll
+        for (int q = 0; q < NS; q++)
+            if (ks[q] >= 0) fma_cols<V, TWO, E, C, C>(a, rr[n - 1 - ks[q]], S[q], acc);
+        if (NS == 0 || ks[NS - 1] >= 0) {
+            int kA = pop_bond(mm, TB0);
+            if (kA >= 0) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << kA), lo, S[0]);
+            while (kA >= 0) {
+                const int kB = pop_bond(mm, TB0);
+                if (kB >= 0) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << kB), lo, S[1]);
+                fma_cols<V, TWO, E, C, C>(a, rr[n - 1 - kA], S[0], acc);
+                if (kB < 0) break;
+                kA = pop_bond(mm, TB0);
+                if (kA >= 0) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << kA), lo, S[0]);
+                fma_cols<V, TWO, E, C, C>(a, rr[n - 1 - kB], S[1], acc);
+            }
         }
         // ---- diagonal and epilogue
 #pragma unroll
@@ -428,10 +491,7 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
             }
             st_out<V, E>(a.out + wb + lo + ((uint32_t)c << LB), o);
         }
-#pragma unroll
-        for (int c = 0; c < C; c++)
-#pragma unroll
-            for (int j = 0; j < E; j++) g[c][j] = gn[c][j];
+        if (more || TWO) cur = gn;
     }
     if constexpr (EPI != EPI_STORE) {
         const double bc = block_sum<kTileThreads>(chk.value(), red);
@@ -466,6 +526,7 @@ struct TileRefArgs {
     int n;
     uint64_t ntiles;
     int xcd_order;
+    uint32_t xcd_res;
 };
 
 __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) {
@@ -508,38 +569,44 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
     const double w_b6 = lane_hi ? rb(6) : 0.0;                    // bond 6 in-flow, columns 0, 2
     const double d_b6 = lane_hi ? 0.0 : rb(6);                    // bond 6 out-flow, columns 1, 3
     Kahan chk, mass;
-    const uint64_t per_xcd = a.ntiles >> 3;
+    const TileWalk walk = tile_walk(a.ntiles, a.xcd_order, a.xcd_res);
     const uint64_t stride = gridDim.x;
-    const uint64_t last = a.ntiles - 1;
     const double h = a.h;
 
-    uint64_t it = blockIdx.x;
-    double g[C][E];
-    if (it < a.ntiles) {
-        const uint64_t base = (tile_of(it, per_xcd, a.xcd_order) << TB0) + wofs;
+    constexpr int NS = TASEP_TILE_SLOTS;
+    auto issue = [&](uint64_t base, double (&dst)[C][E]) {
 #pragma unroll
-        for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), g[c]);
-    }
-    for (; it < a.ntiles; it += stride) {
-        const uint64_t T = tile_of(it, per_xcd, a.xcd_order);
+        for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), dst[c]);
+    };
+    uint64_t it = blockIdx.x, kt = 0;
+    double g[C][E];
+    if (it < a.ntiles) issue((walk.base << TB0) + wofs, g);
+    for (; it < a.ntiles; it += stride, kt++) {
+        const uint64_t T = walk.base + kt * walk.step;
         const uint64_t wb = (T << TB0) + wofs;
         const bool t_lo = T & 1, t_hi = (T >> (tb - 1)) & 1;
-        // ---- issue: first tile-bond stream and the next tile's own vector
+        // ---- issue every global read of this tile in the order it is
+        // added: bond 8 (columns 2, 3 read columns 0, 1 of tile T+1), the
+        // first NS tile bonds, bond n-4, then the next tile's own vector
+        double seam[2][E];
+        if (!t_lo) {
+            ld16(a.x + wb + lo + ((uint64_t)1 << TB0), seam[0]);
+            ld16(a.x + wb + lo + ((uint64_t)1 << TB0) + (1u << LB), seam[1]);
+        }
         uint64_t mm = T & ~(T >> 1) & tmask;
-        int kA = pop_bond(mm, TB0);
-        double A[C][E], B[C][E];
-        {
-            const uint64_t base = wb + (kA >= 0 ? ((uint64_t)1 << kA) : 0);
+        int ks[NS];
+        double S[NS > 1 ? NS : 2][C][E];
 #pragma unroll
-            for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), A[c]);
+        for (int q = 0; q < NS; q++) {
+            ks[q] = pop_bond(mm, TB0);
+            if (ks[q] >= 0) issue(wb + ((uint64_t)1 << ks[q]), S[q]);
         }
+        const bool has_n4 = t_hi && !w0;
+        double s4[C][E];
+        if (has_n4) issue(wb + ((uint64_t)1 << (n - 4)), s4);
         double gn[C][E];
-        {
-            const uint64_t itn = it + stride <= last ? it + stride : it;
-            const uint64_t base = (tile_of(itn, per_xcd, a.xcd_order) << TB0) + wofs;
-#pragma unroll
-            for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), gn[c]);
-        }
+        const bool more = it + stride < a.ntiles;
+        if (more) issue(((T + walk.step) << TB0) + wofs, gn);
         __syncthreads();  // the previous tile's LDS reads are done
 #pragma unroll
         for (int c = 0; c < C; c++) st16(&tile[lds0 + c * LCOL], g[c]);
@@ -608,57 +675,40 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
         for (int j = 0; j < E; j++) Tv[1][j] += rb(LB) * g[2][j];
         // ---- bond 8: columns 2, 3 read columns 0, 1 of tile T+1
         if (!t_lo) {
-            double nb[2][E];
-            ld16(a.x + wb + lo + ((uint64_t)1 << TB0), nb[0]);
-            ld16(a.x + wb + lo + ((uint64_t)1 << TB0) + (1u << LB), nb[1]);
             const double r8 = rb(LB + 1);
 #pragma unroll
             for (int j = 0; j < E; j++) {
-                Tv[2][j] += r8 * nb[0][j];
-                Tv[3][j] += r8 * nb[1][j];
+                Tv[2][j] += r8 * seam[0][j];
+                Tv[3][j] += r8 * seam[1][j];
             }
         }
-        // ---- tile bonds ascending, two streams in flight
-        while (kA >= 0) {
-            const int kB = pop_bond(mm, TB0);
-            {
-                const uint64_t base = wb + (kB >= 0 ? ((uint64_t)1 << kB) : 0);
+        // ---- tile bonds ascending: the first NS as issued, the rest two
+        // streams in flight at a time
+        auto add = [&](int k, const double (&nb)[C][E]) {
+            const double r = rb(k);
 #pragma unroll
-                for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), B[c]);
-            }
-            {
-                const double r = rb(kA);
+            for (int c = 0; c < C; c++)
 #pragma unroll
-                for (int c = 0; c < C; c++)
+                for (int j = 0; j < E; j++) Tv[c][j] += r * nb[c][j];
+        };
 #pragma unroll
-                    for (int j = 0; j < E; j++) Tv[c][j] += r * A[c][j];
-            }
-            if (kB < 0) break;
-            kA = pop_bond(mm, TB0);
-            {
-                const uint64_t base = wb + (kA >= 0 ? ((uint64_t)1 << kA) : 0);
-#pragma unroll
-                for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), A[c]);
-            }
-            {
-                const double r = rb(kB);
-#pragma unroll
-                for (int c = 0; c < C; c++)
-#pragma unroll
-                    for (int j = 0; j < E; j++) Tv[c][j] += r * B[c][j];
+        for (int q = 0; q < NS; q++)
+            if (ks[q] >= 0) add(ks[q], S[q]);
+        if (ks[NS - 1] >= 0) {
+            int kA = pop_bond(mm, TB0);
+            if (kA >= 0) issue(wb + ((uint64_t)1 << kA), S[0]);
+            while (kA >= 0) {
+                const int kB = pop_bond(mm, TB0);
+                if (kB >= 0) issue(wb + ((uint64_t)1 << kB), S[1]);
+                add(kA, S[0]);
+                if (kB < 0) break;
+                kA = pop_bond(mm, TB0);
+                if (kA >= 0) issue(wb + ((uint64_t)1 << kA), S[0]);
+                add(kB, S[1]);
             }
         }
         // ---- bond n-4: chunk w+1 of tile T - 2^(tb-1), 2^(n-4) states away
-        if (t_hi && !w0) {
-            const double r = rb(n - 4);
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                double nb[E];
-                ld16(a.x + wb + ((uint64_t)1 << (n - 4)) + lo + ((uint32_t)c << LB), nb);
-#pragma unroll
-                for (int j = 0; j < E; j++) Tv[c][j] += r * nb[j];
-            }
-        }
+        if (has_n4) add(n - 4, s4);
         // ---- bonds n-3, n-2: chunks w+1, w+2 of this tile
         if (w0 && !w1) {
             const double r = rb(n - 3);
@@ -693,10 +743,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
             }
             st16(a.out + wb + lo + ((uint32_t)c << LB), o);
         }
+        if (more) {
 #pragma unroll
-        for (int c = 0; c < C; c++)
+            for (int c = 0; c < C; c++)
 #pragma unroll
-            for (int j = 0; j < E; j++) g[c][j] = gn[c][j];
+                for (int j = 0; j < E; j++) g[c][j] = gn[c][j];
+        }
     }
     const double bc = block_sum<kTileThreads>(chk.value(), red);
     const double bm = block_sum<kTileThreads>(mass.value(), red);

@@ -293,6 +293,34 @@ def test_xcd_order_does_not_change_results():
     assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("case", ["f32-horner", "f32-staged-tv", "f64-horner", "f64-ref"])
+def test_tile_orders_bitwise_equal(case):
+    """Every tile order (grid, XCD eighths, per-XCD sweep, device sweep) and
+    grid size gives the same bits: each state's row is summed in a fixed
+    order whichever workgroup owns its tile.  n=22 with 256 workgroups gives
+    every workgroup several tiles, so the sweeps are really taken."""
+    n = 22
+    prec, kind = case.split("-", 1)
+    rng = np.random.default_rng(23)
+    r = rng.uniform(0.2, 1.5, n + 1)
+    tv = dict(type=(np.arange(n + 1) % 3).astype(np.int32), amp=rng.uniform(0, 0.5, n + 1),
+              period=rng.uniform(0.5, 2, n + 1), offset=rng.uniform(0, 1, n + 1), c=np.ones(n + 1))
+    out = []
+    for order, blocks in ((1, 0), (0, 256), (1, 256), (2, 256), (3, 256), (2, 0), (3, 0)):
+        with T.Engine(n, prec, "ref" if kind == "ref" else "rk4", "mf") as e:
+            e.set_option(T.OPT_XCD_ORDER, order)
+            e.set_option(T.OPT_TILE_BLOCKS, blocks)
+            if kind == "staged-tv":
+                e.set_time_variant(tv)
+            else:
+                e.set_rates(r)
+            e.set_x0_uniform()
+            e.run(0.0, 0.01, 12, patience=-1)
+            out.append(e.get_x())
+    for o in out[1:]:
+        assert _bits_equal(out[0], o)
+
+
 @pytest.mark.parametrize("prec,tiled", [("f64", 2), ("f64", 0), ("f32", 2)])
 def test_horner_equals_staged_rk4(prec, tiled):
     n = 15
