@@ -2,9 +2,12 @@
 //
 // Used to calibrate hardware traffic counters (rocprofv3 --pmc FETCH_SIZE /
 // WRITE_SIZE read back per kernel dispatch) against bytes that are known
-// exactly, and to measure the practical HBM bandwidth of the three stream
-// shapes of the generator passes: read-only, copy (1 read + 1 write) and
-// triad (2 reads + 1 write).
+// exactly (copy4, triad4, read4: one 16-byte access per lane and
+// iteration), and to measure the practical HBM bandwidth of the three stream
+// shapes of the generator passes -- read-only, copy (1 read + 1 write) and
+// triad (2 reads + 1 write) -- in their fastest form found on MI355X
+// (copy_nt, triad_nt, read_nt: 4 or 8 independent 16-byte loads per lane in
+// flight, non-temporal loads and stores).
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
@@ -39,6 +42,48 @@ __global__ void read4(const float4 *__restrict__ a, float *out, size_t n) {
         s += x.x + x.y + x.z + x.w;
     }
     if (s == 1234.5f) out[0] = s;  // keeps the loads; never true for zero data
+}
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Workgroups walk contiguous chunks of U * blockDim 16-byte elements; n must
+// be a multiple of the chunk.
+template <int U>
+__global__ void copy_nt(const f4 *__restrict__ a, f4 *__restrict__ b, size_t n) {
+    const size_t chunk = (size_t)U * blockDim.x;
+    for (size_t base = blockIdx.x * chunk; base < n; base += (size_t)gridDim.x * chunk) {
+        f4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = __builtin_nontemporal_load(a + base + u * blockDim.x + threadIdx.x);
+#pragma unroll
+        for (int u = 0; u < U; u++) __builtin_nontemporal_store(v[u], b + base + u * blockDim.x + threadIdx.x);
+    }
+}
+
+template <int U>
+__global__ void triad_nt(const f4 *__restrict__ a, const f4 *__restrict__ c, f4 *__restrict__ b, size_t n) {
+    const size_t chunk = (size_t)U * blockDim.x;
+    for (size_t base = blockIdx.x * chunk; base < n; base += (size_t)gridDim.x * chunk) {
+        f4 v[U], w[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            v[u] = __builtin_nontemporal_load(a + base + u * blockDim.x + threadIdx.x);
+            w[u] = __builtin_nontemporal_load(c + base + u * blockDim.x + threadIdx.x);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) __builtin_nontemporal_store(v[u] + 0.5f * w[u], b + base + u * blockDim.x + threadIdx.x);
+    }
+}
+
+template <int U>
+__global__ void read_nt(const f4 *__restrict__ a, float *out, size_t n) {
+    const size_t chunk = (size_t)U * blockDim.x;
+    f4 s = {0, 0, 0, 0};
+    for (size_t base = blockIdx.x * chunk; base < n; base += (size_t)gridDim.x * chunk) {
+#pragma unroll
+        for (int u = 0; u < U; u++) s += __builtin_nontemporal_load(a + base + u * blockDim.x + threadIdx.x);
+    }
+    if (s.x + s.y + s.z + s.w == 1234.5f) out[0] = 1;  // keeps the loads; never true for zero data
 }
 
 int main() {
@@ -76,6 +121,29 @@ int main() {
     CK(hipEventSynchronize(e1));
     CK(hipEventElapsedTime(&ms, e0, e1));
     printf("read  1 GiB read: %.3f ms  %.0f GB/s\n", ms / reps, 1.0 * bytes / (ms / reps * 1e-3) / 1e9);
+    // fastest forms: 8192 workgroups of 256 lanes, chunks of 1024 / 2048 elements divide n4 = 2^26
+    const int g2 = 8192;
+    const f4 *fa = (const f4 *)a, *fc = (const f4 *)c;
+    f4 *fb = (f4 *)b;
+    copy_nt<4><<<g2, 256>>>(fa, fb, n4);
+    CK(hipEventRecord(e0));
+    for (int r = 0; r < reps; r++) copy_nt<4><<<g2, 256>>>(fa, fb, n4);
+    CK(hipEventRecord(e1));
+    CK(hipEventSynchronize(e1));
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    printf("copy_nt  (4 in flight, non-temporal): %.3f ms  %.0f GB/s\n", ms / reps, 2.0 * bytes / (ms / reps * 1e-3) / 1e9);
+    CK(hipEventRecord(e0));
+    for (int r = 0; r < reps; r++) triad_nt<4><<<g2, 256>>>(fa, fc, fb, n4);
+    CK(hipEventRecord(e1));
+    CK(hipEventSynchronize(e1));
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    printf("triad_nt (4 in flight, non-temporal): %.3f ms  %.0f GB/s\n", ms / reps, 3.0 * bytes / (ms / reps * 1e-3) / 1e9);
+    CK(hipEventRecord(e0));
+    for (int r = 0; r < reps; r++) read_nt<8><<<g2, 256>>>(fa, o, n4);
+    CK(hipEventRecord(e1));
+    CK(hipEventSynchronize(e1));
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    printf("read_nt  (8 in flight, non-temporal): %.3f ms  %.0f GB/s\n", ms / reps, 1.0 * bytes / (ms / reps * 1e-3) / 1e9);
     CK(hipFree(a));
     CK(hipFree(b));
     CK(hipFree(c));
