@@ -28,16 +28,18 @@
 //   entry (n-1)       wave bit 2: chunk w-4 via LDS
 // The diagonal splits the same way (static, per-lane, per-wave, per-tile).
 // Every global neighbour access is a contiguous stream at most 2^(n-4)
-// states away; only active tile bonds are read (two streams in flight), and
-// the per-tile uniform work is shared by 16 states per lane.
+// states away; only active tile bonds are read, and the per-tile uniform
+// work is shared by 16 states per lane.
 //
-// Memory-level parallelism: the workgroup is persistent over its tiles; the
-// next tile's own vector is prefetched while the current one is computed,
-// the self vector and the first neighbour stream are issued before the LDS
-// phase, and the remaining neighbour streams are double-buffered.
+// Memory-level parallelism: the workgroup is persistent over its tiles.
+// Before a tile's LDS phase it issues every global read the tile needs, in
+// the order they are consumed -- the seam, bond n-4, the first
+// TASEP_TILE_SLOTS tile bonds, the self vector and the next tile's own
+// vector -- so their latency overlaps the LDS work; further tile bonds are
+// read two at a time afterwards.
 //
-// Tiles are walked in an XCD-aware order: workgroup g runs on XCD g % 8 and
-// takes its tiles from a contiguous eighth of the tile range.
+// Tiles are walked in an XCD-aware order (tile_walk): workgroup g runs on
+// XCD g % 8, and each XCD sweeps its own eighth of the tile range in order.
 #pragma once
 
 #include "device_common.hpp"
