@@ -65,12 +65,27 @@ __device__ __forceinline__ double ref_row_mf(const Lattice g, uint64_t s,
     return T;
 }
 
+// a / 6 correctly rounded, without the general division sequence.
+// q = RN(a * y) with y = RN(1/6) is within one ulp of a/6, the remainder
+// r = a - 6q is exact in one FMA, and RN(q + r * y) is then the correctly
+// rounded quotient (Markstein's correction theorem: y within half an ulp of
+// 1/b, q faithful, no underflow).  Zeros, subnormals and |a| < 2^-960, where
+// the products could underflow, take the division itself.  Checked against
+// a / 6.0 on 5e8 random and near-midpoint operands.
+__device__ __forceinline__ double div6(double a) {
+    constexpr double y = 1.0 / 6.0;
+    if (!(fabs(a) >= 0x1p-960)) return a / 6.0;
+    const double q = a * y;
+    const double r = __builtin_fma(-6.0, q, a);
+    return __builtin_fma(r, y, q);
+}
+
 // The reference update for one state: all four stages equal k = h*T
 // (src/lib/solv.c:12-13 reads the unmodified x), combined with the
 // reference's rounding (src/masterEquation.c:640).
 __device__ __forceinline__ double ref_delta(double h, double T) {
     const double k = h * T;
-    return (((k + 2.0 * k) + 2.0 * k) + k) / 6.0;
+    return div6(((k + 2.0 * k) + 2.0 * k) + k);
 }
 
 // ---------------------------------------------------------------------------

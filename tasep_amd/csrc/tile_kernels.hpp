@@ -568,6 +568,19 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
     }
     const double w_b0 = (l & 1) ? 0.0 : rb(0);                    // bond 0 in-flow (odd j)
     const double d_b0 = (l & 1) ? rb(0) : 0.0;                    // bond 0 out-flow (even j)
+    // leading part of the diagonal, the same for every tile: 0 - beta - alpha
+    // - bonds 0..5 (subtracting an inactive bond's 0 is exact)
+    double d_lane[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        double d = 0.0;
+        if (j & 1) d -= beta;
+        if (!w2) d -= alpha;
+        if (j == 0) d -= d_b0;
+#pragma unroll
+        for (int q = 0; q < 5; q++) d -= dl5[q];
+        d_lane[j] = d;
+    }
     const double w_b6 = lane_hi ? rb(6) : 0.0;                    // bond 6 in-flow, columns 0, 2
     const double d_b6 = lane_hi ? 0.0 : rb(6);                    // bond 6 out-flow, columns 1, 3
     Kahan chk, mass;
@@ -615,8 +628,29 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
         __syncthreads();
         const uint64_t hout = ~T & (T >> 1) & tmask;
         double Tv[C][E];
-        // ---- diagonal (reference subtraction order), then entry and d x(s)
+        // ---- diagonal (reference subtraction order: the lane part, bonds
+        // 6..8, tile bonds ascending, n-4, n-3, n-2), then entry and d x(s)
         {
+            double dg[C][E];
+#pragma unroll
+            for (int c = 0; c < C; c++)
+#pragma unroll
+                for (int j = 0; j < E; j++) {
+                    double d = d_lane[j];
+                    if (c & 1) d -= d_b6;
+                    if (c == 2) d -= rb(LB);
+                    if (c < 2 && t_lo) d -= rb(LB + 1);
+                    dg[c][j] = d;
+                }
+            for (uint64_t m = hout; m; m &= m - 1) {
+                const double r = rb(TB0 + __builtin_ctzll(m));
+#pragma unroll
+                for (int c = 0; c < C; c++)
+#pragma unroll
+                    for (int j = 0; j < E; j++) dg[c][j] -= r;
+            }
+            const double r4 = rb(n - 4), r3 = rb(n - 3), r2 = rb(n - 2);
+            const bool o4 = !t_hi && w0, o3 = !w0 && w1, o2 = !w1 && w2;
             double lv[E];
             const double wa = w2 ? alpha : 0.0;
             const uint32_t src = w2 ? lds0 - 4 * C * LCOL : lds0;
@@ -625,19 +659,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
                 ld16(&tile[src + c * LCOL], lv);
 #pragma unroll
                 for (int j = 0; j < E; j++) {
-                    double d = 0.0;
-                    if (j & 1) d -= beta;
-                    if (!w2) d -= alpha;
-                    if (j == 0) d -= d_b0;
-#pragma unroll
-                    for (int q = 0; q < 5; q++) d -= dl5[q];
-                    if (c & 1) d -= d_b6;
-                    if (c == 2) d -= rb(LB);
-                    if (c < 2 && t_lo) d -= rb(LB + 1);
-                    for (uint64_t m = hout; m; m &= m - 1) d -= rb(TB0 + __builtin_ctzll(m));
-                    if (!t_hi && w0) d -= rb(n - 4);
-                    if (!w0 && w1) d -= rb(n - 3);
-                    if (!w1 && w2) d -= rb(n - 2);
+                    double d = dg[c][j];
+                    if (o4) d -= r4;
+                    if (o3) d -= r3;
+                    if (o2) d -= r2;
                     double T0 = 0.0;
                     T0 += wa * lv[j];
                     T0 += d * g[c][j];
@@ -732,17 +757,20 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
                 for (int j = 0; j < E; j++) Tv[c][j] += r * lv[j];
             }
         }
-        // ---- the reference update, checksum and mass
+        // ---- the reference update, checksum and mass (compensated over
+        // the per-column sums)
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            double o[E];
+            double o[E], cs = 0.0, ms = 0.0;
 #pragma unroll
             for (int j = 0; j < E; j++) {
                 const double d = ref_delta(h, Tv[c][j]);
                 o[j] = g[c][j] + d;
-                chk.add(fabs(d));
-                mass.add(o[j]);
+                cs += fabs(d);
+                ms += o[j];
             }
+            chk.add(cs);
+            mass.add(ms);
             st16(a.out + wb + lo + ((uint32_t)c << LB), o);
         }
         if (more) {
