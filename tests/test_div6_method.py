@@ -1,11 +1,12 @@
-"""The reference update divides by 6 with IEEE division (src/masterEquation.c:640).
-The device kernels replace that division by an FMA-corrected product
-(div6 in tasep_amd/csrc/generator.hpp): q = RN(a*y), y = RN(1/6),
-r = fma(-6, q, a), q' = fma(r, y, q).  This test restates that sequence in C
-on the CPU (same IEEE binary64 operations) and checks it against a / 6.0,
-bit for bit, on random operands over the exponent range the kernel uses it
-for and on operands next to products 6*q (near-midpoint quotients).  The GPU
-parity tests then pin the kernels themselves bitwise against the oracle."""
+"""The reference update divides by 6 with IEEE division
+(src/masterEquation.c:640).  The device kernels replace that division by an
+FMA-corrected product (div6 in tasep_amd/csrc/generator.hpp): q = RN(a*y),
+y = RN(1/6), r = fma(-6, q, a), q' = copysign(fma(r, y, q), a).  This test
+restates that sequence in C on the CPU (same IEEE binary64 operations) and
+checks it against a / 6.0, bit for bit, on random operands over the whole
+exponent range, on operands next to products 6*q (near-midpoint quotients)
+and on signed zeros and extremes.  The GPU parity tests then pin the kernels
+themselves bitwise against the oracle."""
 import os
 import shutil
 import subprocess
@@ -22,10 +23,10 @@ static uint64_t s = 0x9E3779B97F4A7C15ull;
 static uint64_t xr(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return s; }
 static double q6(double a) {
     const double y = 1.0 / 6.0;
-    if (!(fabs(a) >= 0x1p-960)) return a / 6.0;
+    if (fabs(a) < 0x1p-960 && a != 0.0) return a / 6.0;
     double q = a * y;
     double r = fma(-6.0, q, a);
-    return fma(r, y, q);
+    return copysign(fma(r, y, q), a);
 }
 static long check(double a) {
     double t = a / 6.0, u = q6(a);
