@@ -233,6 +233,55 @@ __global__ __launch_bounds__(1024) void k_lds_run(const LdsArgs a) {
     const bool diag_once = INTEG == 0 && !TV && D <= (uint32_t)nt;
     double d_once = 0.0;
     if (INTEG == 0 && !TV && diag_once && (uint32_t)t < D) d_once = ref_diag_lds(n, (uint32_t)t, qd);
+    // reference loop (stored matrix, or constant rates) with at most one
+    // state per lane: the row's nonzeros once, in registers, in row order
+    // (a row has 1 + b_top + !b_0 + #in-bonds <= 3 + n/2 <= 9 entries for
+    // n <= 13).  The matrix-free row leaves out the terms that do not pull
+    // into s, which add 0 * x = +-0 and leave T unchanged (T is never -0).
+    constexpr bool kRowRegs = INTEG == 2 || (INTEG == 0 && !TV);
+    constexpr int kRow = 3 + kLdsMaxN / 2;
+    const bool row_once = kRowRegs && D <= (uint32_t)nt;
+    uint32_t rcol[kRowRegs ? kRow : 1];
+    double rval[kRowRegs ? kRow : 1];
+    int rlen = 0;
+    if constexpr (INTEG == 2) {
+        if (row_once && (uint32_t)t < D) {
+            const uint64_t b0 = a.rowPtr[t];
+            rlen = (int)(a.rowPtr[t + 1] - b0);
+#pragma unroll
+            for (int j = 0; j < kRow; j++) {
+                rcol[j] = j < rlen ? (uint32_t)a.colInd[b0 + j] : 0u;
+                rval[j] = j < rlen ? a.val[b0 + j] : 0.0;
+            }
+        }
+    } else if constexpr (kRowRegs) {
+        if (row_once) {
+#pragma unroll
+            for (int j = 0; j < kRow; j++) {
+                rcol[j] = 0u;
+                rval[j] = 0.0;
+            }
+            if ((uint32_t)t < D) {
+                // compact the row: term number rlen goes to slot rlen
+                const uint32_t s0 = (uint32_t)t, top = 1u << (n - 1);
+                auto put = [&](bool on, uint32_t col, double v) {
+#pragma unroll
+                    for (int i = 0; i < kRow; i++)
+                        if (on && i == rlen) {
+                            rcol[i] = col;
+                            rval[i] = v;
+                        }
+                    rlen += on ? 1 : 0;
+                };
+                put((s0 & top) != 0, s0 ^ top, qd.alpha);
+                put(true, s0, d_once);
+                put(!(s0 & 1u), s0 ^ 1u, qd.beta);
+#pragma unroll
+                for (int k = 0; k < kLdsBonds; k++)
+                    put(k < n - 1 && ((s0 >> k) & 3u) == 1u, s0 + (1u << k), qd.bk[k]);
+            }
+        }
+    }
     int par = 0;
     V *cur = xs, *nxt = bA;  // the reference loop's state alternates
     // end of a step's last pass: checksum partial and next step's rates
@@ -249,7 +298,12 @@ __global__ __launch_bounds__(1024) void k_lds_run(const LdsArgs a) {
             auto gx = [&](uint32_t j) { return cur[j]; };
             for (uint32_t s = t; s < D; s += nt) {
                 double T;
-                if constexpr (INTEG == 2) {
+                if (row_once) {
+                    T = 0.0;
+#pragma unroll
+                    for (int j = 0; j < kRow; j++)
+                        if (j < rlen) T += rval[j] * (double)cur[rcol[j]];
+                } else if constexpr (INTEG == 2) {
                     T = 0.0;
                     const uint64_t e = a.rowPtr[s + 1];
                     for (uint64_t j = a.rowPtr[s]; j < e; j++) T += a.val[j] * (double)cur[a.colInd[j]];
