@@ -53,8 +53,13 @@ __global__ __launch_bounds__(kBlock) void k_ref_step(const RefArgs a) {
 // (every lane loading 8 consecutive entries), then each lane sums its own
 // row from LDS in CSR order (src/lib/solv.c:8-13) -- where lanes that each
 // walked their own row in global memory touched 64 different cache lines
-// per load.  A row that crosses a chunk keeps its partial sum.
+// per load.  A row that crosses a chunk keeps its partial sum.  Within a
+// chunk a lane first gathers the x values of all its entries (independent
+// loads, in flight together) and then adds the products in CSR order.
 constexpr int kCsrChunk = 512;
+// Entries of one row: 1 + b_top + !b_0 + #in-bonds <= 3 + ceil((n-1)/2),
+// at most 19 for n <= 32.
+constexpr int kCsrRowMax = 20;
 
 __global__ __launch_bounds__(kWave) void k_ref_csr_wave(const RefArgs a) {
     if (a.st->stopped) return;
@@ -74,12 +79,35 @@ __global__ __launch_bounds__(kWave) void k_ref_csr_wave(const RefArgs a) {
         for (uint64_t base = seg0; base < seg1; base += kCsrChunk) {
             const uint64_t cend = base + kCsrChunk < seg1 ? base + kCsrChunk : seg1;
             __syncthreads();  // the previous chunk is consumed
-            for (uint64_t e = base + t; e < cend; e += kWave) {
-                col[e - base] = a.colInd[e];
-                val[e - base] = a.val[e];
+            // all loads in flight together (lanes past the chunk's end reload
+            // its last entry and store nothing)
+            constexpr int U = kCsrChunk / kWave;
+            uint64_t cu[U];
+            double vu[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint64_t e = base + t + (uint64_t)u * kWave;
+                const uint64_t ec = e < cend ? e : cend - 1;
+                cu[u] = a.colInd[ec];
+                vu[u] = a.val[ec];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint64_t e = base + t + (uint64_t)u * kWave;
+                if (e < cend) {
+                    col[e - base] = cu[u];
+                    val[e - base] = vu[u];
+                }
             }
             __syncthreads();
-            for (; j < je && j < cend; j++) T += val[j - base] * a.x[col[j - base]];
+            const uint64_t jc = je < cend ? je : cend;             // this lane's entries: [j, jc)
+            double xv[kCsrRowMax];
+#pragma unroll
+            for (int r = 0; r < kCsrRowMax; r++) xv[r] = j + r < jc ? a.x[col[j + r - base]] : 0.0;
+#pragma unroll
+            for (int r = 0; r < kCsrRowMax; r++)
+                if (j + r < jc) T += val[j + r - base] * xv[r];
+            if (j < jc) j = jc;
         }
         if (s < a.dim) {
             const double d = ref_delta(a.h, T);
