@@ -206,19 +206,22 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_finish(const BatchInst 
 }
 
 // ===========================================================================
-// Tile form of the batched pass.  A 1024-lane workgroup takes one tile of
-// 2^TL consecutive states of one instance (TL = 13 FP32, 12 FP64; 32 KiB of
-// LDS) and lane t holds local states t + 1024 j.  Bonds inside the tile come
+// Tile form of the batched pass.  A 128-lane workgroup takes one tile of
+// 2^TL consecutive states of one instance (TL = 10 FP32, 9 FP64; 4 KiB of
+// LDS) and lane t holds local states t + 128 j.  Small workgroups keep many
+// tiles in flight per CU, which hides the latency of each tile's chain of
+// loads (config-4 batch, 1000 steps: 1024 lanes 494 ms, 512 510, 256 445,
+// 128 431, 64 478).  Bonds inside the tile come
 // from LDS (bonds < 6 branch-free per lane, bonds 6..TL-2 wave-uniform);
 // the bonds that cross tiles are uniform over the workgroup: the seam bond
 // TL-1 (local bit TL-1, tile bit 0: half a tile of T+1), the tile bonds
 // (tile T + 2^(k-TL), two streams in flight) and the entry (tile T - 2^(n-1-TL)),
 // each a coalesced stream read only when it pulls into the tile.
 // ===========================================================================
-constexpr int kBatchTileThreads = 1024;
+constexpr int kBatchTileThreads = 128;
 template <class V> struct BatchTileGeom;
-template <> struct BatchTileGeom<float> { static constexpr int TL = 13, J = 8; };
-template <> struct BatchTileGeom<double> { static constexpr int TL = 12, J = 4; };
+template <> struct BatchTileGeom<float> { static constexpr int TL = 10, J = 8; };
+template <> struct BatchTileGeom<double> { static constexpr int TL = 9, J = 4; };
 
 template <class V>
 __global__ __launch_bounds__(kBatchTileThreads, 8) void k_batch_tile(const BatchArgs<V> a) {
