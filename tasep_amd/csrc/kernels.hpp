@@ -56,14 +56,14 @@ __global__ __launch_bounds__(kBlock) void k_ref_step(const RefArgs a) {
 // per load.  A row that crosses a chunk keeps its partial sum.  Within a
 // chunk a lane first gathers the x values of all its entries (independent
 // loads, in flight together) and then adds the products in CSR order.
-constexpr int kCsrChunk = 512;
-// Entries of one row: 1 + b_top + !b_0 + #in-bonds <= 3 + ceil((n-1)/2),
-// at most 19 for n <= 32.
-constexpr int kCsrRowMax = 20;
+constexpr int kCsrChunk = 512;  // 6 KiB of LDS per wave (256: 693 ms, 512: 637 ms for config 2)
+// Row entries gathered together (a row has 1 + b_top + !b_0 + #in-bonds
+// <= 3 + ceil((n-1)/2) entries: 6.75 on average at n = 20).
+constexpr int kCsrGather = 8;
 
 __global__ __launch_bounds__(kWave) void k_ref_csr_wave(const RefArgs a) {
     if (a.st->stopped) return;
-    __shared__ uint64_t col[kCsrChunk];
+    __shared__ uint32_t col[kCsrChunk];                           // state indices < 2^32 (n <= 32)
     __shared__ double val[kCsrChunk];
     __shared__ double red[1];
     const int t = threadIdx.x;
@@ -82,13 +82,13 @@ __global__ __launch_bounds__(kWave) void k_ref_csr_wave(const RefArgs a) {
             // all loads in flight together (lanes past the chunk's end reload
             // its last entry and store nothing)
             constexpr int U = kCsrChunk / kWave;
-            uint64_t cu[U];
+            uint32_t cu[U];
             double vu[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const uint64_t e = base + t + (uint64_t)u * kWave;
                 const uint64_t ec = e < cend ? e : cend - 1;
-                cu[u] = a.colInd[ec];
+                cu[u] = (uint32_t)a.colInd[ec];
                 vu[u] = a.val[ec];
             }
 #pragma unroll
@@ -100,13 +100,17 @@ __global__ __launch_bounds__(kWave) void k_ref_csr_wave(const RefArgs a) {
                 }
             }
             __syncthreads();
-            const uint64_t jc = je < cend ? je : cend;             // this lane's entries: [j, jc)
-            double xv[kCsrRowMax];
+            // this lane's entries in the chunk: [j, jc) (empty when its row
+            // starts in a later chunk or has ended)
+            const uint64_t jc = je < cend ? je : cend;
+            for (uint64_t q = j; q < jc; q += kCsrGather) {
+                double xv[kCsrGather];
 #pragma unroll
-            for (int r = 0; r < kCsrRowMax; r++) xv[r] = j + r < jc ? a.x[col[j + r - base]] : 0.0;
+                for (int r = 0; r < kCsrGather; r++) xv[r] = q + r < jc ? a.x[col[q + r - base]] : 0.0;
 #pragma unroll
-            for (int r = 0; r < kCsrRowMax; r++)
-                if (j + r < jc) T += val[j + r - base] * xv[r];
+                for (int r = 0; r < kCsrGather; r++)
+                    if (q + r < jc) T += val[q + r - base] * xv[r];
+            }
             if (j < jc) j = jc;
         }
         if (s < a.dim) {

@@ -95,6 +95,23 @@ def test_ref_loop_bit_exact_vs_oracle(n, gen):
     assert abs(info.last_checksum - oinfo.last_checksum) <= 1e-14 * oinfo.last_checksum
 
 
+def test_ref_loop_csr_rows_across_matrix_chunks():
+    """n=20: groups of 64 rows hold ~430 matrix entries on average, so many
+    groups span two 512-entry chunks of the stored-matrix kernel and rows
+    continue across the chunk boundary.  Stored matrix and matrix-free rows
+    must stay bitwise equal."""
+    n = 20
+    r = np.random.default_rng(20).uniform(0.1, 2.0, n + 1)
+    xs = []
+    for gen in ("csr", "mf"):
+        with T.Engine(n, "f64", "ref", gen) as e:
+            e.set_rates(r)
+            e.set_x0_uniform()
+            e.run(0.0, 0.01, 12, patience=-1)
+            xs.append(e.get_x())
+    assert _bits_equal(xs[0], xs[1])
+
+
 def test_ref_early_stop_config1_uniform_is_stationary():
     """n=10, alpha=beta=1/2, gamma=1 from the uniform state: the reference
     leaves after 101 sub-threshold steps (index 100, t = 1.01)."""
