@@ -70,12 +70,14 @@ __device__ __forceinline__ double ref_row_mf(const Lattice g, uint64_t s,
 // r = a - 6q is exact in one FMA, and RN(q + r * y) is then the correctly
 // rounded quotient (Markstein's correction theorem: y within half an ulp of
 // 1/b, q faithful, no underflow).  Nonzero |a| < 2^-960, where the products
-// could underflow, takes the division itself; a signed zero keeps its sign
-// through copysign (the sequence alone would turn -0 into +0).  Checked
-// against a / 6.0 on 5e8 random and near-midpoint operands.
+// could underflow, and |a| > 2^1000 (infinities included) take the division
+// itself; a signed zero keeps its sign through copysign (the sequence alone
+// would turn -0 into +0).  Checked against a / 6.0 on 5e8 random and
+// near-midpoint operands.
 __device__ __forceinline__ double div6(double a) {
     constexpr double y = 1.0 / 6.0;
-    if (fabs(a) < 0x1p-960 && a != 0.0) return a / 6.0;
+    const double m = fabs(a);
+    if ((m < 0x1p-960 || m > 0x1p1000) && a != 0.0) return a / 6.0;
     const double q = a * y;
     const double r = __builtin_fma(-6.0, q, a);
     return copysign(__builtin_fma(r, y, q), a);

@@ -23,7 +23,8 @@ static uint64_t s = 0x9E3779B97F4A7C15ull;
 static uint64_t xr(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return s; }
 static double q6(double a) {
     const double y = 1.0 / 6.0;
-    if (fabs(a) < 0x1p-960 && a != 0.0) return a / 6.0;
+    const double m = fabs(a);
+    if ((m < 0x1p-960 || m > 0x1p1000) && a != 0.0) return a / 6.0;
     double q = a * y;
     double r = fma(-6.0, q, a);
     return copysign(fma(r, y, q), a);
@@ -49,7 +50,8 @@ int main(void) {
             bad += check(b);
         }
     }
-    bad += check(0.0) + check(-0.0) + check(0x1p-1074) + check(0x1p-960) + check(0x1.fffffffffffffp+1023);
+    bad += check(0.0) + check(-0.0) + check(0x1p-1074) + check(0x1p-960) + check(0x1.fffffffffffffp+1023)
+         + check(INFINITY) + check(-INFINITY) + check(0x1p1000) + check(-0x1.8p999);
     printf("%ld %ld\n", n, bad);
     return 0;
 }
