@@ -185,7 +185,11 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish_step(RunState *st, co
 // sc*X[s], where sc = RunState::scale renormalises FP32 vectors by the FP64
 // mass of the previous step.  All arithmetic is FP64; V is the storage type.
 // ===========================================================================
-enum { EPI_STORE = 0, EPI_HORNER = 1, EPI_RK4 = 2 };
+// EPI_STORE: out = self + c * (L G).  EPI_HORNER: x += c * (L G) with the
+// step's checksum and mass.  EPI_RK4: the staged step's combination from
+// stored k1, k2, k3.  EPI_STAGE / EPI_RK4ACC (tiled passes): the staged step
+// with materialised stage inputs and a running k-sum (see tile_kernels.hpp).
+enum { EPI_STORE = 0, EPI_HORNER = 1, EPI_RK4 = 2, EPI_STAGE = 3, EPI_RK4ACC = 4 };
 
 template <class V>
 struct PassArgs {

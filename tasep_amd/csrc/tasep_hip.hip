@@ -520,7 +520,7 @@ static void launch_tile(tasep_ctx *c, TileArgs<V> a, const char *name, double by
     prof_begin(c);
     hipLaunchKernelGGL((k_tile_pass<V, TWO, EPI, SELF>), dim3(g), dim3(kTileThreads), 0, c->stream, a);
     prof_end(c, name, bytes);
-    if (EPI != EPI_STORE) launch_finish(c, (uint32_t)g);
+    if (EPI == EPI_HORNER || EPI == EPI_RK4 || EPI == EPI_RK4ACC) launch_finish(c, (uint32_t)g);
 }
 
 template <class V>
@@ -542,25 +542,33 @@ static void rk4_horner_tile(tasep_ctx *c, double h) {
     launch_tile<V, false, EPI_HORNER>(c, a, "horner_p4", 3 * e);
 }
 
+// Staged RK4 on tiles (time-variant rates) with materialised stage inputs:
+// each pass gathers one vector, y_i, and writes y_{i+1} = x + c_i k_i and
+// the running sum acc = k1 + 2 k2 + 2 k3 (same roundings as stored k's).
+//   s1: k1 = h L(t) x          y2 = x + k1/2   acc = k1
+//   s2: k2 = h L(t+h/2) y2     y3 = x + k2/2   acc += 2 k2
+//   s3: k3 = h L(t+h/2) y3     y4 = x + k3     acc += 2 k3
+//   s4: k4 = h L(t+h) y4       x' = x + (acc + k4) / 6
 template <class V>
 static void rk4_staged_tile(tasep_ctx *c, double h, uint64_t j) {
     V *x = (V *)c->buf[c->cur ^ (int)(j & 1)];
     V *xn = (V *)c->buf[c->cur ^ (int)((j + 1) & 1)];
-    V *k1 = (V *)c->buf[2], *k2 = (V *)c->buf[3], *k3 = (V *)c->buf[4];
+    V *ya = (V *)c->buf[2], *yb = (V *)c->buf[3], *acc = (V *)c->buf[4];
     TileArgs<V> a = tile_base<V>(c);
     const double e = (double)sizeof(V);
     a.x = x;
-    a.gx = x;
     a.c = (V)h;
-    a.use_self = 0;
-    a.rset = 0; a.out = k1;
-    launch_tile<V, false, EPI_STORE>(c, a, "rk4_s1", 2 * e);
-    a.rset = 1; a.gk = k1; a.ck = (V)0.5; a.out = k2;
-    launch_tile<V, true, EPI_STORE>(c, a, "rk4_s23", 3 * e);
-    a.rset = 1; a.gk = k2; a.ck = (V)0.5; a.out = k3;
-    launch_tile<V, true, EPI_STORE>(c, a, "rk4_s23", 3 * e);
-    a.rset = 2; a.gk = k3; a.ck = (V)1.0; a.out = xn; a.k1 = k1; a.k2 = k2; a.k3 = k3;
-    launch_tile<V, true, EPI_RK4>(c, a, "rk4_s4", 5 * e);
+    a.use_self = 1;
+    a.acc_in = acc;
+    a.acc_out = acc;
+    a.rset = 0; a.gx = x; a.out = ya; a.cy = (V)0.5; a.stage = 0;
+    launch_tile<V, false, EPI_STAGE, true>(c, a, "rk4_s1", 3 * e);
+    a.rset = 1; a.gx = ya; a.out = yb; a.cy = (V)0.5; a.stage = 1;
+    launch_tile<V, false, EPI_STAGE>(c, a, "rk4_s23", 5 * e);
+    a.rset = 1; a.gx = yb; a.out = ya; a.cy = (V)1.0; a.stage = 2;
+    launch_tile<V, false, EPI_STAGE>(c, a, "rk4_s23", 5 * e);
+    a.rset = 2; a.gx = ya; a.out = xn;
+    launch_tile<V, false, EPI_RK4ACC>(c, a, "rk4_s4", 4 * e);
 }
 
 // Tiles need n >= b + 1 (two tiles); below n = 18 the one-lane-per-state

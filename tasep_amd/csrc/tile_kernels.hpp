@@ -136,6 +136,12 @@ struct TileArgs {
     V *out;
     const V *k1, *k2, *k3;
     V c;               // out = self + c * (L G)
+    // EPI_STAGE: k = c (L G); out = self + cy k; acc_out = k (stage 0) or
+    // acc_in + 2 k.  EPI_RK4ACC: x_new = self + (acc_in + c (L G)) / 6.
+    const V *acc_in;
+    V *acc_out;
+    V cy;
+    int stage;
     int use_self;      // STORE: add X[s] (or G[s] when SELF)
     const double *rates;
     int rset;
@@ -321,7 +327,12 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
             if (ks[q] >= 0) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << ks[q]), lo, S[q]);
         }
         V xs[C][E];
-        if constexpr (EPI != EPI_STORE || !SELF) load_cols_x<V, E, C, LB>(a.x, wb, lo, xs);
+        if constexpr (!SELF) load_cols_x<V, E, C, LB>(a.x, wb, lo, xs);
+        constexpr bool kAcc = EPI == EPI_STAGE || EPI == EPI_RK4ACC;
+        V ac[kAcc ? C : 1][E];                                     // running k-sum
+        if constexpr (kAcc) {
+            if (EPI == EPI_RK4ACC || a.stage > 0) load_cols_x<V, E, C, LB>(a.acc_in, wb, lo, ac);
+        }
         // (the last tile reloads itself in the staged passes, which keeps
         // cur = gn a plain register renaming there)
         const bool more = it + stride < a.ntiles;
@@ -467,10 +478,26 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
                     const V self = SELF ? g[c][j] : (use_self ? xs[c][j] : (V)0);
                     o[j] = fma(a.c, fma(-d[j], g[c][j], acc[c][j]), self);
                 }
+            } else if constexpr (EPI == EPI_STAGE) {
+                // k rounded as the stored stages round it (fma with +0), the
+                // next stage input and the running sum k1 + 2 k2 + 2 k3
+                V ao[E];
+#pragma unroll
+                for (int j = 0; j < E; j++) {
+                    const V self = SELF ? g[c][j] : xs[c][j];
+                    const V kk = fma(a.c, fma(-d[j], g[c][j], acc[c][j]), (V)0);
+                    o[j] = fma(a.cy, kk, self);
+                    ao[j] = a.stage > 0 ? (V)(ac[c][j] + (V)2 * kk) : kk;
+                }
+                st_out<V, E>(a.acc_out + wb + lo + ((uint32_t)c << LB), ao);
             } else {
                 V dl[E];
 #pragma unroll
                 for (int j = 0; j < E; j++) dl[j] = a.c * fma(-d[j], g[c][j], acc[c][j]);
+                if constexpr (EPI == EPI_RK4ACC) {
+#pragma unroll
+                    for (int j = 0; j < E; j++) dl[j] = (ac[c][j] + dl[j]) / (V)6;
+                }
                 if constexpr (EPI == EPI_RK4) {
                     V q1[E], q2[E], q3[E];
                     const uint64_t s = wb + lo + ((uint32_t)c << LB);
@@ -495,7 +522,7 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
         }
         if (more || TWO) cur = gn;
     }
-    if constexpr (EPI != EPI_STORE) {
+    if constexpr (EPI == EPI_HORNER || EPI == EPI_RK4 || EPI == EPI_RK4ACC) {
         const double bc = block_sum<kTileThreads>(chk.value(), red);
         const double bm = block_sum<kTileThreads>(mass.value(), red);
         store_partials(a.pchk, a.pmass, bc, bm);
