@@ -510,17 +510,17 @@ static TileLaunch tile_launch(const tasep_ctx *c, const void *kernel, int &slots
     return L;
 }
 
-template <class V, bool TWO, int EPI, bool SELF = false>
+template <class V, int EPI, bool SELF = false>
 static void launch_tile(tasep_ctx *c, TileArgs<V> a, const char *name, double bytes) {
     static int slots = 0;
-    const TileLaunch L = tile_launch(c, (const void *)k_tile_pass<V, TWO, EPI, SELF>, slots, a.ntiles);
+    const TileLaunch L = tile_launch(c, (const void *)k_tile_pass<V, EPI, SELF>, slots, a.ntiles);
     const uint64_t g = L.grid;
     a.xcd_order = L.order;
     a.xcd_res = L.res;
     prof_begin(c);
-    hipLaunchKernelGGL((k_tile_pass<V, TWO, EPI, SELF>), dim3(g), dim3(kTileThreads), 0, c->stream, a);
+    hipLaunchKernelGGL((k_tile_pass<V, EPI, SELF>), dim3(g), dim3(kTileThreads), 0, c->stream, a);
     prof_end(c, name, bytes);
-    if (EPI == EPI_HORNER || EPI == EPI_RK4 || EPI == EPI_RK4ACC) launch_finish(c, (uint32_t)g);
+    if (EPI == EPI_HORNER || EPI == EPI_RK4ACC) launch_finish(c, (uint32_t)g);
 }
 
 template <class V>
@@ -533,13 +533,13 @@ static void rk4_horner_tile(tasep_ctx *c, double h) {
     a.x = x;
     a.use_self = 1;
     a.gx = x; a.out = u; a.c = (V)(0.25 * h);
-    launch_tile<V, false, EPI_STORE, true>(c, a, "horner_p1", 2 * e);
+    launch_tile<V, EPI_STORE, true>(c, a, "horner_p1", 2 * e);
     a.gx = u; a.out = v; a.c = (V)(h / 3.0);
-    launch_tile<V, false, EPI_STORE>(c, a, "horner_p23", 3 * e);
+    launch_tile<V, EPI_STORE>(c, a, "horner_p23", 3 * e);
     a.gx = v; a.out = u; a.c = (V)(0.5 * h);
-    launch_tile<V, false, EPI_STORE>(c, a, "horner_p23", 3 * e);
+    launch_tile<V, EPI_STORE>(c, a, "horner_p23", 3 * e);
     a.gx = u; a.out = x; a.c = (V)h;
-    launch_tile<V, false, EPI_HORNER>(c, a, "horner_p4", 3 * e);
+    launch_tile<V, EPI_HORNER>(c, a, "horner_p4", 3 * e);
 }
 
 // Staged RK4 on tiles (time-variant rates) with materialised stage inputs:
@@ -562,13 +562,13 @@ static void rk4_staged_tile(tasep_ctx *c, double h, uint64_t j) {
     a.acc_in = acc;
     a.acc_out = acc;
     a.rset = 0; a.gx = x; a.out = ya; a.cy = (V)0.5; a.stage = 0;
-    launch_tile<V, false, EPI_STAGE, true>(c, a, "rk4_s1", 3 * e);
+    launch_tile<V, EPI_STAGE, true>(c, a, "rk4_s1", 3 * e);
     a.rset = 1; a.gx = ya; a.out = yb; a.cy = (V)0.5; a.stage = 1;
-    launch_tile<V, false, EPI_STAGE>(c, a, "rk4_s23", 5 * e);
+    launch_tile<V, EPI_STAGE>(c, a, "rk4_s23", 5 * e);
     a.rset = 1; a.gx = yb; a.out = ya; a.cy = (V)1.0; a.stage = 2;
-    launch_tile<V, false, EPI_STAGE>(c, a, "rk4_s23", 5 * e);
+    launch_tile<V, EPI_STAGE>(c, a, "rk4_s23", 5 * e);
     a.rset = 2; a.gx = ya; a.out = xn;
-    launch_tile<V, false, EPI_RK4ACC>(c, a, "rk4_s4", 4 * e);
+    launch_tile<V, EPI_RK4ACC>(c, a, "rk4_s4", 4 * e);
 }
 
 // Tiles need n >= b + 1 (two tiles); below n = 18 the one-lane-per-state

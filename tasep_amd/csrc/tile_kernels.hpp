@@ -53,14 +53,11 @@ constexpr int kTileThreads = 512;
 #define TASEP_TILE_WAVES 2
 #endif
 constexpr int kTileWaves = kTileThreads / kWave;
-// Tile-bond streams issued per tile before its LDS phase (registers:
-// 16 per stream and lane, twice that in the staged passes); a tile with
-// more active tile bonds reads the rest two at a time afterwards.
+// Tile-bond streams issued per tile before its LDS phase (16 registers per
+// stream and lane); a tile with more active tile bonds reads the rest two at
+// a time afterwards.
 #ifndef TASEP_TILE_SLOTS
 #define TASEP_TILE_SLOTS 2
-#endif
-#ifndef TASEP_TILE_SLOTS_TWO
-#define TASEP_TILE_SLOTS_TWO 0
 #endif
 
 template <class V> struct TileGeom;
@@ -129,12 +126,9 @@ __device__ __forceinline__ void st_out(V *p, const V (&o)[E]) {
 
 template <class V>
 struct TileArgs {
-    const V *gx;       // gathered vector
-    const V *gk;       // second gathered vector (staged RK4)
-    V ck;              // G = gx + ck * gk
+    const V *gx;       // gathered vector G
     const V *x;        // state, self term
     V *out;
-    const V *k1, *k2, *k3;
     V c;               // out = self + c * (L G)
     // EPI_STAGE: k = c (L G); out = self + cy k; acc_out = k (stage 0) or
     // acc_in + 2 k.  EPI_RK4ACC: x_new = self + (acc_in + c (L G)) / 6.
@@ -154,41 +148,25 @@ struct TileArgs {
 };
 
 
-// Stage-input values of one lane as loaded: R columns 2^LB states apart of
-// gx and, for the staged RK4 passes, of gk.  Loading and combining are split
-// so that a load can be issued long before its value is used.
-template <class V, bool TWO, int R, int E>
+// Stage-input values of one lane: R columns 2^LB states apart.
+template <class V, int R, int E>
 struct Cols {
     V g[R][E];
-    V k[TWO ? R : 1][E];
 };
 
-template <class V, bool TWO, int E, int R, int LB>
-__device__ __forceinline__ void issue_cols(const TileArgs<V> &a, uint64_t base, uint32_t lo,
-                                           Cols<V, TWO, R, E> &s) {
+template <class V, int E, int R, int LB>
+__device__ __forceinline__ void issue_cols(const TileArgs<V> &a, uint64_t base, uint32_t lo, Cols<V, R, E> &s) {
 #pragma unroll
     for (int c = 0; c < R; c++) ld16(a.gx + base + lo + ((uint32_t)c << LB), s.g[c]);
-    if constexpr (TWO) {
-#pragma unroll
-        for (int c = 0; c < R; c++) ld16(a.gk + base + lo + ((uint32_t)c << LB), s.k[c]);
-    }
-}
-
-// Column c, register j of the stage input G = gx [+ ck * gk].
-template <class V, bool TWO, int R, int E>
-__device__ __forceinline__ V col_value(const TileArgs<V> &a, const Cols<V, TWO, R, E> &s, int c, int j) {
-    if constexpr (TWO) return fma(a.ck, s.k[c][j], s.g[c][j]);
-    else return s.g[c][j];
 }
 
 // acc[c0 + c] += r * G[c] for the R loaded columns.
-template <class V, bool TWO, int E, int R, int C>
-__device__ __forceinline__ void fma_cols(const TileArgs<V> &a, V r, const Cols<V, TWO, R, E> &s,
-                                         V (&acc)[C][E], int c0 = 0) {
+template <class V, int E, int R, int C>
+__device__ __forceinline__ void fma_cols(V r, const Cols<V, R, E> &s, V (&acc)[C][E], int c0 = 0) {
 #pragma unroll
     for (int c = 0; c < R; c++)
 #pragma unroll
-        for (int j = 0; j < E; j++) acc[c0 + c][j] = fma(r, col_value(a, s, c, j), acc[c0 + c][j]);
+        for (int j = 0; j < E; j++) acc[c0 + c][j] = fma(r, s.g[c][j], acc[c0 + c][j]);
 }
 
 template <class V, int E, int C, int LB>
@@ -236,7 +214,7 @@ __device__ __forceinline__ int pop_bond(uint64_t &m, int LB0) {
     return k;
 }
 
-template <class V, bool TWO, int EPI, bool SELF>
+template <class V, int EPI, bool SELF>
 __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(const TileArgs<V> a) {
     using G = TileGeom<V>;
     constexpr int E = G::E, e = G::e, LB = G::LB, C = G::C, TB0 = G::TB0;
@@ -290,41 +268,31 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
     const uint64_t stride = gridDim.x;
 
     uint64_t it = blockIdx.x, kt = 0;
-    constexpr int NS = TWO ? TASEP_TILE_SLOTS_TWO : TASEP_TILE_SLOTS;
-    using Str = Cols<V, TWO, C, E>;
+    constexpr int NS = TASEP_TILE_SLOTS;
+    using Str = Cols<V, C, E>;
     Str cur, gn;                                                   // own vectors: this tile, the next
     if (it < a.ntiles)  // prologue: own vector of the first tile
-        issue_cols<V, TWO, E, C, LB>(a, (walk.base << TB0) + wofs, lo, cur);
+        issue_cols<V, E, C, LB>(a, (walk.base << TB0) + wofs, lo, cur);
     for (; it < a.ntiles; it += stride, kt++) {
         const uint64_t T = walk.base + kt * walk.step;
         const uint64_t wb = (T << TB0) + wofs;                     // wave base (uniform)
         const bool t_lo = T & 1, t_hi = (T >> (tb - 1)) & 1;
-        // own vector: combined here in the staged passes (frees the raw
-        // registers), at the LDS store otherwise (the first tile's streams
-        // are issued before its own vector has arrived)
-        V g[C][E];
-        if constexpr (TWO) {
-#pragma unroll
-            for (int c = 0; c < C; c++)
-#pragma unroll
-                for (int j = 0; j < E; j++) g[c][j] = col_value(a, cur, c, j);
-        }
         // ---- issue every global read of this tile in the order it is used:
         // bond LB+1 (columns 2, 3 read columns 0, 1 of tile T+1), bond n-4
         // (per wave), the first NS tile bonds, the self vector, then the next
         // tile's own vector; all are in flight during the LDS phase
-        Cols<V, TWO, 2, E> seam;
-        if (!t_lo) issue_cols<V, TWO, E, 2, LB>(a, wb + ((uint64_t)1 << TB0), lo, seam);
+        Cols<V, 2, E> seam;
+        if (!t_lo) issue_cols<V, E, 2, LB>(a, wb + ((uint64_t)1 << TB0), lo, seam);
         const bool has_n4 = t_hi && !w0;
         Str s4;
-        if (!TWO && has_n4) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << (n - 4)), lo, s4);
+        if (has_n4) issue_cols<V, E, C, LB>(a, wb + ((uint64_t)1 << (n - 4)), lo, s4);
         uint64_t mm = T & ~(T >> 1) & tmask;
         int ks[NS > 0 ? NS : 1] = {-1};
         Str S[NS > 1 ? NS : 2];
 #pragma unroll
         for (int q = 0; q < NS; q++) {
             ks[q] = pop_bond(mm, TB0);
-            if (ks[q] >= 0) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << ks[q]), lo, S[q]);
+            if (ks[q] >= 0) issue_cols<V, E, C, LB>(a, wb + ((uint64_t)1 << ks[q]), lo, S[q]);
         }
         V xs[C][E];
         if constexpr (!SELF) load_cols_x<V, E, C, LB>(a.x, wb, lo, xs);
@@ -333,17 +301,11 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
         if constexpr (kAcc) {
             if (EPI == EPI_RK4ACC || a.stage > 0) load_cols_x<V, E, C, LB>(a.acc_in, wb, lo, ac);
         }
-        // (the last tile reloads itself in the staged passes, which keeps
-        // cur = gn a plain register renaming there)
         const bool more = it + stride < a.ntiles;
-        if (more || TWO) issue_cols<V, TWO, E, C, LB>(a, ((more ? T + walk.step : T) << TB0) + wofs, lo, gn);
-        // ---- own vector to LDS
-        if constexpr (!TWO) {
-#pragma unroll
-            for (int c = 0; c < C; c++)
-#pragma unroll
-                for (int j = 0; j < E; j++) g[c][j] = col_value(a, cur, c, j);
-        }
+        if (more) issue_cols<V, E, C, LB>(a, ((T + walk.step) << TB0) + wofs, lo, gn);
+        // ---- own vector to LDS (waited for only here: the first tile's
+        // streams are issued before it has arrived)
+        const V(&g)[C][E] = cur.g;
         __syncthreads();  // the previous tile's LDS reads are done
 #pragma unroll
         for (int c = 0; c < C; c++) st16(&tile[lds0 + c * LCOL], g[c]);
@@ -427,27 +389,22 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
             for (int j = 0; j < E; j += 2) acc[c][j] = fma(beta, g[c][j + 1], acc[c][j]);
         // ---- global streams, in issue order; tile bonds beyond the first NS
         // two at a time
-        if (!t_lo) fma_cols<V, TWO, E, 2, C>(a, r_c2, seam, acc, 2);
-        if (has_n4) {
-            // the staged passes (two vectors per stream) issue it here: at
-            // 256 registers they cannot hold it through the LDS phase
-            if constexpr (TWO) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << (n - 4)), lo, s4);
-            fma_cols<V, TWO, E, C, C>(a, r_n4, s4, acc);
-        }
+        if (!t_lo) fma_cols<V, E, 2, C>(r_c2, seam, acc, 2);
+        if (has_n4) fma_cols<V, E, C, C>(r_n4, s4, acc);
 #pragma unroll
         for (int q = 0; q < NS; q++)
-            if (ks[q] >= 0) fma_cols<V, TWO, E, C, C>(a, rr[n - 1 - ks[q]], S[q], acc);
+            if (ks[q] >= 0) fma_cols<V, E, C, C>(rr[n - 1 - ks[q]], S[q], acc);
         if (NS == 0 || ks[NS - 1] >= 0) {
             int kA = pop_bond(mm, TB0);
-            if (kA >= 0) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << kA), lo, S[0]);
+            if (kA >= 0) issue_cols<V, E, C, LB>(a, wb + ((uint64_t)1 << kA), lo, S[0]);
             while (kA >= 0) {
                 const int kB = pop_bond(mm, TB0);
-                if (kB >= 0) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << kB), lo, S[1]);
-                fma_cols<V, TWO, E, C, C>(a, rr[n - 1 - kA], S[0], acc);
+                if (kB >= 0) issue_cols<V, E, C, LB>(a, wb + ((uint64_t)1 << kB), lo, S[1]);
+                fma_cols<V, E, C, C>(rr[n - 1 - kA], S[0], acc);
                 if (kB < 0) break;
                 kA = pop_bond(mm, TB0);
-                if (kA >= 0) issue_cols<V, TWO, E, C, LB>(a, wb + ((uint64_t)1 << kA), lo, S[0]);
-                fma_cols<V, TWO, E, C, C>(a, rr[n - 1 - kB], S[1], acc);
+                if (kA >= 0) issue_cols<V, E, C, LB>(a, wb + ((uint64_t)1 << kA), lo, S[0]);
+                fma_cols<V, E, C, C>(rr[n - 1 - kB], S[1], acc);
             }
         }
         // ---- diagonal and epilogue
@@ -498,16 +455,6 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
 #pragma unroll
                     for (int j = 0; j < E; j++) dl[j] = (ac[c][j] + dl[j]) / (V)6;
                 }
-                if constexpr (EPI == EPI_RK4) {
-                    V q1[E], q2[E], q3[E];
-                    const uint64_t s = wb + lo + ((uint32_t)c << LB);
-                    ld16(a.k1 + s, q1);
-                    ld16(a.k2 + s, q2);
-                    ld16(a.k3 + s, q3);
-#pragma unroll
-                    for (int j = 0; j < E; j++)
-                        dl[j] = (((q1[j] + (V)2 * q2[j]) + (V)2 * q3[j]) + dl[j]) / (V)6;
-                }
                 double cs = 0, ms = 0;
 #pragma unroll
                 for (int j = 0; j < E; j++) {
@@ -520,9 +467,9 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
             }
             st_out<V, E>(a.out + wb + lo + ((uint32_t)c << LB), o);
         }
-        if (more || TWO) cur = gn;
+        if (more) cur = gn;
     }
-    if constexpr (EPI == EPI_HORNER || EPI == EPI_RK4 || EPI == EPI_RK4ACC) {
+    if constexpr (EPI == EPI_HORNER || EPI == EPI_RK4ACC) {
         const double bc = block_sum<kTileThreads>(chk.value(), red);
         const double bm = block_sum<kTileThreads>(mass.value(), red);
         store_partials(a.pchk, a.pmass, bc, bm);

@@ -88,20 +88,24 @@ def test_batch_lds_instances_equal_single_engine_bitwise():
 
 
 def test_batch_graph_and_streams_do_not_change_results():
+    """Graph replay and the second stream change nothing, bit for bit, under
+    each schedule.  The schedule is fixed per group: the autotuner's choice
+    between two schedules of similar speed can differ from run to run, and
+    the two schedules sum rows in different orders."""
     ns = [9, 13, 14, 16, 15, 14]
     steps = 21   # two graph replays of 8 steps and 5 direct steps
     runs = []
-    for opts in (dict(graph_steps=0, concurrent=0), dict(graph_steps=8, concurrent=1),
-                 dict(graph_steps=8, concurrent=0), dict(graph_steps=8, schedule=2),
+    for opts in (dict(graph_steps=0, concurrent=0, schedule=1), dict(graph_steps=8, concurrent=1, schedule=1),
+                 dict(graph_steps=8, concurrent=0, schedule=1), dict(graph_steps=8, schedule=2),
                  dict(graph_steps=0, schedule=2)):
         b, _ = _setup(ns, 3, "f32", **opts)
         with b:
             b.run(0.0, 0.01, steps)
             runs.append([b.get_x(i) for i in range(len(ns))])
-    for r in runs[1:3]:
+    for r in runs[1:3]:                    # lanes: graph replay, streams
         for a, c in zip(runs[0], r):
             assert np.array_equal(a, c)
-    for a, c in zip(runs[3], runs[4]):     # tiles: graph replay changes nothing
+    for a, c in zip(runs[3], runs[4]):     # tiles: graph replay
         assert np.array_equal(a, c)
 
 
