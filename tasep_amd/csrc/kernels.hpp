@@ -148,7 +148,9 @@ __global__ __launch_bounds__(kFinishThreads) void k_window_sums(const double *__
 
 __global__ void k_window_apply(RunState *st, const double *__restrict__ tot, uint32_t m) {
     if (threadIdx.x != 0) return;
-    for (uint32_t i = 0; i < m && !st->stopped; i++) advance_state(st, tot[2 * i], tot[2 * i + 1]);
+    RunState s = *st;                  // the m steps in registers, one write back
+    for (uint32_t i = 0; i < m && !s.stopped; i++) advance_state(&s, tot[2 * i], tot[2 * i + 1]);
+    *st = s;
 }
 
 // End of `steps` steps: sum the last final kernel's `nblocks` block
