@@ -253,6 +253,58 @@ def test_rk4_bernoulli_product_is_stationary_large(prec):
     assert info.last_checksum <= (1e-4 if prec == "f32" else 1e-12)
 
 
+def _bernoulli_product(n, a):
+    """x_s = a^N(s) (1-a)^(n-N(s)), N(s) = occupied sites: a Kronecker
+    product of n identical site marginals."""
+    x = np.array([1.0])
+    site = np.array([1.0 - a, a])
+    for _ in range(n):
+        x = np.kron(x, site)
+    return x
+
+
+@pytest.mark.parametrize("kind", ["horner", "staged-tv"])
+def test_full_size_n28_bernoulli_product_is_stationary(kind):
+    """The n=28 benchmark size (2^28 states, FP32 tiles with the per-XCD
+    sweep and pre-issued streams): alpha + beta = 1, gamma = 1 keeps the
+    Bernoulli(alpha) product measure stationary.  The staged passes run on
+    time-variant rates that are constant in time (linear, zero slope)."""
+    n, a = 28, 0.3
+    r = np.array([a] + [1.0] * (n - 1) + [1 - a])
+    with T.Engine(n, "f32", "rk4", "mf") as e:
+        if kind == "horner":
+            e.set_rates(r)
+        else:
+            e.set_time_variant(dict(type=np.full(n + 1, 2, np.int32), amp=np.zeros(n + 1),
+                                    period=np.ones(n + 1), offset=np.zeros(n + 1), c=r))
+        e.set_x(_bernoulli_product(n, a))
+        info = e.run(0.0, 0.01, 6, patience=-1)
+        y, cov, _, mass = e.moments()
+    assert info.steps_done == 6
+    assert np.max(np.abs(y - a)) <= 1e-6
+    assert np.max(np.abs(cov - np.diag(np.diag(cov)))) <= 1e-6
+    assert abs(mass - 1.0) <= 1e-5
+
+
+def test_full_size_n28_ref_tiles_bitwise_vs_per_state_kernel():
+    """The bit-exact reference loop at the n=28 benchmark size: the tiled
+    kernel (per-XCD sweep, pre-issued streams, hoisted diagonal, division by
+    6 through the FMA correction) equals the one-lane-per-state kernel bit
+    for bit."""
+    n = 28
+    r = np.random.default_rng(28).uniform(0.1, 2.0, n + 1)
+    xs, infos = [], []
+    for tiled in (1, 0):
+        with T.Engine(n, "f64", "ref", "mf") as e:
+            e.set_option(T.OPT_TILED, tiled)
+            e.set_rates(r)
+            e.set_x0_uniform()
+            infos.append(e.run(0.0, 0.01, 3, patience=-1))
+            xs.append(e.get_x())
+    assert _bits_equal(xs[0], xs[1])
+    assert abs(infos[0].last_checksum - infos[1].last_checksum) <= 1e-14 * infos[1].last_checksum
+
+
 def test_converged_rk4_matches_dehp():
     from oracle import dehp
     n = 12
