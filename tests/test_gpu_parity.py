@@ -286,6 +286,25 @@ def test_full_size_n28_bernoulli_product_is_stationary(kind):
     assert abs(mass - 1.0) <= 1e-5
 
 
+@pytest.mark.parametrize("n,prec", [(32, "f32"), (28, "f64")])
+def test_full_size_uniform_is_stationary(n, prec):
+    """alpha = beta = 1/2, gamma = 1: the uniform distribution is exactly
+    stationary (every state's out-rate equals its in-rate), at the largest
+    benchmark size (n=32, 2^32 FP32 states, 16 GiB per vector) and at n=28
+    in FP64."""
+    r = np.array([0.5] + [1.0] * (n - 1) + [0.5])
+    with T.Engine(n, prec, "rk4", "mf") as e:
+        e.set_rates(r)
+        e.set_x0_uniform()
+        info = e.run(0.0, 0.01, 3, patience=-1)
+        y, cov, _, mass = e.moments()
+    tol = 1e-6 if prec == "f32" else 1e-12
+    assert info.steps_done == 3
+    assert np.max(np.abs(y - 0.5)) <= tol
+    assert np.max(np.abs(cov - np.diag(np.diag(cov)))) <= tol
+    assert abs(mass - 1.0) <= (1e-5 if prec == "f32" else 1e-12)
+
+
 def test_full_size_n28_ref_tiles_bitwise_vs_per_state_kernel():
     """The bit-exact reference loop at the n=28 benchmark size: the tiled
     kernel (per-XCD sweep, pre-issued streams, hoisted diagonal, division by
