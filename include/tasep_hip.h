@@ -243,10 +243,12 @@ int tasep_batch_window_marginals(tasep_batch *batch, int i, double *W);
  *   TASEP_BATCH_OPT_CONCURRENT  [1]  the LDS-resident instances run on a
  *                                    second stream beside the shared passes
  *   TASEP_BATCH_OPT_SCHEDULE    [0]  shared passes: 1 one lane per state,
- *                                    2 tiles of 2^13 (FP32) / 2^12 (FP64)
+ *                                    2 tiles of 2^10 (FP32) / 2^9 (FP64)
  *                                    states in LDS, 0 = timed on the first
  *                                    run and the faster kept (get_option
- *                                    then reports the choice) */
+ *                                    then reports the choice; the two
+ *                                    agree to rounding, so pin 1 or 2 for
+ *                                    bitwise run-to-run reproducibility) */
 #define TASEP_BATCH_OPT_GRAPH_STEPS 1
 #define TASEP_BATCH_OPT_LDS         2
 #define TASEP_BATCH_OPT_CONCURRENT  3
