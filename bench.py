@@ -420,7 +420,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--config", type=int, default=0,
-                    help="run one of the five baseline configurations at full size instead")
+                    help="run one of the five benchmark workloads (1..5) at full size instead")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -433,7 +433,7 @@ def main():
     if args.config:
         fn = {1: config_1, 2: config_2, 3: config_3, 4: config_4, 5: config_5}.get(args.config)
         if fn is None:
-            sys.exit(f"configuration {args.config} is not available")
+            sys.exit(f"workload {args.config} is not available")
         print(json.dumps({"config": args.config, "result": fn(T, ev)}), flush=True)
         return
     n, h, K, W = args.n, args.h, args.steps, args.warmup

@@ -83,8 +83,7 @@ def load_library():
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C tasep_amd` "
-                           "(or __graft_entry__.build())")
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C tasep_amd`")
     L = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     L.tasep_strerror.restype = C.c_char_p

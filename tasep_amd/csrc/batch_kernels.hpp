@@ -210,7 +210,8 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_finish(const BatchInst 
 // 2^TL consecutive states of one instance (TL = 10 FP32, 9 FP64; 4 KiB of
 // LDS) and lane t holds local states t + 128 j.  Small workgroups keep many
 // tiles in flight per CU, which hides the latency of each tile's chain of
-// loads (config-4 batch, 1000 steps: 1024 lanes 494 ms, 512 510, 256 445,
+// loads (512 instances, n in 8..18, half with time-variant rates, 1000
+// FP32 steps: 1024 lanes 494 ms, 512 510, 256 445,
 // 128 431, 64 478).  Bonds inside the tile come
 // from LDS (bonds < 6 branch-free per lane, bonds 6..TL-2 wave-uniform);
 // the bonds that cross tiles are uniform over the workgroup: the seam bond

@@ -56,7 +56,8 @@ __global__ __launch_bounds__(kBlock) void k_ref_step(const RefArgs a) {
 // per load.  A row that crosses a chunk keeps its partial sum.  Within a
 // chunk a lane first gathers the x values of all its entries (independent
 // loads, in flight together) and then adds the products in CSR order.
-constexpr int kCsrChunk = 512;  // 6 KiB of LDS per wave (256: 693 ms, 512: 637 ms for config 2)
+// 6 KiB of LDS per wave (n=20, 20,000 steps: 256-entry chunks 693 ms, 512 637 ms)
+constexpr int kCsrChunk = 512;
 // Row entries gathered together (a row has 1 + b_top + !b_0 + #in-bonds
 // <= 3 + ceil((n-1)/2) entries: 6.75 on average at n = 20).
 constexpr int kCsrGather = 8;

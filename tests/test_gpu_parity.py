@@ -112,7 +112,7 @@ def test_ref_loop_csr_rows_across_matrix_chunks():
     assert _bits_equal(xs[0], xs[1])
 
 
-def test_ref_early_stop_config1_uniform_is_stationary():
+def test_ref_early_stop_n10_uniform_is_stationary():
     """n=10, alpha=beta=1/2, gamma=1 from the uniform state: the reference
     leaves after 101 sub-threshold steps (index 100, t = 1.01)."""
     n = 10
@@ -127,8 +127,9 @@ def test_ref_early_stop_config1_uniform_is_stationary():
     assert np.all(x == 2.0 ** -10)
 
 
-def test_ref_config1_from_empty_lattice_vs_oracle():
-    """Configuration 1 proper: p[0] = 1 (empty lattice), t = 0..100."""
+def test_ref_n10_from_empty_lattice_vs_oracle():
+    """n=10, alpha = beta = 1/2, gamma = 1, from the empty lattice
+    (p[0] = 1), t = 0..100 in 10,000 steps."""
     n = 10
     r = np.array([0.5] + [1.0] * (n - 1) + [0.5])
     h = oracle.step_h(100.0, 0.0, 10000)
@@ -561,8 +562,9 @@ def test_lds_workgroup_size_does_not_change_results(threads):
 
 
 def test_lds_ref_early_stop_and_chunks_match_streaming():
-    """Config-1 rates from the empty lattice: 10000 steps (early exit armed,
-    not reached) in one call and in 7 resumed pieces, LDS vs streaming."""
+    """n=10 homogeneous rates (alpha = beta = 1/2) from the empty lattice:
+    10000 steps (early exit armed, not reached) in one call and in 7 resumed
+    pieces, LDS vs streaming."""
     n = 10
     r = np.array([0.5] + [1.0] * (n - 1) + [0.5])
     h = oracle.step_h(100.0, 0.0, 10000)
