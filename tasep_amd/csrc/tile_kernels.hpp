@@ -34,9 +34,10 @@
 // Memory-level parallelism: the workgroup is persistent over its tiles.
 // Before a tile's LDS phase it issues every global read the tile needs, in
 // the order they are consumed -- the seam, bond n-4, the first
-// TASEP_TILE_SLOTS tile bonds, the self vector and the next tile's own
-// vector -- so their latency overlaps the LDS work; further tile bonds are
-// read two at a time afterwards.
+// TASEP_TILE_SLOTS tile bonds, the self vector (and the running k-sum of
+// the staged passes) and the next tile's own vector -- so their latency
+// overlaps the LDS work; further tile bonds are read two at a time
+// afterwards.
 //
 // Tiles are walked in an XCD-aware order (tile_walk): workgroup g runs on
 // XCD g % 8, and each XCD sweeps its own eighth of the tile range in order.
