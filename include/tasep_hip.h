@@ -183,7 +183,10 @@ int tasep_step_profile(tasep_ctx *ctx, double t0, double h, int reps,
  *   TASEP_OPT_GRAPH        [0] runs of at least two chunks (a chunk is
  *                              ~2^26 state-updates, 4..4096 steps) replay a
  *                              HIP graph of one chunk's launches (measured
- *                              no faster than direct launches on MI355X) */
+ *                              no faster than direct launches on MI355X)
+ *   TASEP_OPT_STORE        [0] output stores of the tiled passes: 0 by
+ *                              lattice size, 1 non-temporal, 2 ordinary
+ *                              (the results are the same bits) */
 #define TASEP_OPT_TILED        1
 #define TASEP_OPT_XCD_ORDER    2
 #define TASEP_OPT_RK4_STAGED   3
@@ -191,6 +194,7 @@ int tasep_step_profile(tasep_ctx *ctx, double t0, double h, int reps,
 #define TASEP_OPT_LDS          5
 #define TASEP_OPT_LDS_THREADS  6
 #define TASEP_OPT_GRAPH        7
+#define TASEP_OPT_STORE        8
 int tasep_set_option(tasep_ctx *ctx, int key, int64_t value);
 
 /* The context's HIP stream (hipStream_t) and device pointer of the current

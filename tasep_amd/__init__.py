@@ -38,7 +38,7 @@ EXPORTS = [
     "tasep_batch_window_marginals", "tasep_batch_get_option",
 ]
 OPT_TILED, OPT_XCD_ORDER, OPT_RK4_STAGED, OPT_TILE_BLOCKS, OPT_LDS, OPT_LDS_THREADS = 1, 2, 3, 4, 5, 6
-OPT_GRAPH = 7
+OPT_GRAPH, OPT_STORE = 7, 8
 BATCH_OPT_GRAPH_STEPS, BATCH_OPT_LDS, BATCH_OPT_CONCURRENT, BATCH_OPT_SCHEDULE = 1, 2, 3, 4
 
 

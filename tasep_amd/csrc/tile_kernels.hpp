@@ -89,10 +89,9 @@ __device__ __forceinline__ void st16(double *__restrict__ p, const double (&o)[2
 // Streaming (non-temporal) variants for vectors touched once per pass: the
 // self-term read of x and the output write.  Keeping them out of the way in
 // L2 leaves its capacity to the gathered vector, which neighbouring tiles
-// re-read.
-#ifndef TASEP_NT_OUT
-#define TASEP_NT_OUT 1
-#endif
+// re-read.  The output's policy is chosen per launch (nt_out): a vector
+// that the memory-side cache can keep until the next pass reads it is
+// written with ordinary stores.
 #ifndef TASEP_NT_SELF
 #define TASEP_NT_SELF 1
 #endif
@@ -120,8 +119,8 @@ __device__ __forceinline__ void ld_self(const V *p, V (&o)[E]) {
     else ld16(p, o);
 }
 template <class V, int E>
-__device__ __forceinline__ void st_out(V *p, const V (&o)[E]) {
-    if constexpr (TASEP_NT_OUT) st16_nt(p, o);
+__device__ __forceinline__ void st_out(V *p, const V (&o)[E], int nt) {
+    if (nt) st16_nt(p, o);
     else st16(p, o);
 }
 
@@ -146,6 +145,7 @@ struct TileArgs {
     uint64_t ntiles;
     int xcd_order;     // tile order (tile_walk)
     uint32_t xcd_res;  // workgroups resident per XCD (order 2)
+    int nt_out;        // outputs written non-temporally
 };
 
 
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
                     o[j] = fma(a.cy, kk, self);
                     ao[j] = a.stage > 0 ? (V)(ac[c][j] + (V)2 * kk) : kk;
                 }
-                st_out<V, E>(a.acc_out + wb + lo + ((uint32_t)c << LB), ao);
+                st_out<V, E>(a.acc_out + wb + lo + ((uint32_t)c << LB), ao, a.nt_out);
             } else {
                 V dl[E];
 #pragma unroll
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
                 chk.add(cs);
                 mass.add(ms);
             }
-            st_out<V, E>(a.out + wb + lo + ((uint32_t)c << LB), o);
+            st_out<V, E>(a.out + wb + lo + ((uint32_t)c << LB), o, a.nt_out);
         }
         if (more) cur = gn;
     }
@@ -504,6 +504,7 @@ struct TileRefArgs {
     uint64_t ntiles;
     int xcd_order;
     uint32_t xcd_res;
+    int nt_out;         // next state written non-temporally
 };
 
 __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) {
@@ -746,7 +747,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
             }
             chk.add(cs);
             mass.add(ms);
-            st16(a.out + wb + lo + ((uint32_t)c << LB), o);
+            st_out<double, E>(a.out + wb + lo + ((uint32_t)c << LB), o, a.nt_out);
         }
         if (more) {
 #pragma unroll

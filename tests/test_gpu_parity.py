@@ -384,10 +384,11 @@ def test_xcd_order_does_not_change_results():
 
 @pytest.mark.parametrize("case", ["f32-horner", "f32-staged-tv", "f64-horner", "f64-ref"])
 def test_tile_orders_bitwise_equal(case):
-    """Every tile order (grid, XCD eighths, per-XCD sweep, device sweep) and
-    grid size gives the same bits: each state's row is summed in a fixed
-    order whichever workgroup owns its tile.  n=22 with 256 workgroups gives
-    every workgroup several tiles, so the sweeps are really taken."""
+    """Every tile order (grid, XCD eighths, per-XCD sweep, device sweep),
+    grid size and output store policy gives the same bits: each state's row
+    is summed in a fixed order whichever workgroup owns its tile.  n=22 with
+    256 workgroups gives every workgroup several tiles, so the sweeps are
+    really taken."""
     n = 22
     prec, kind = case.split("-", 1)
     rng = np.random.default_rng(23)
@@ -395,10 +396,12 @@ def test_tile_orders_bitwise_equal(case):
     tv = dict(type=(np.arange(n + 1) % 3).astype(np.int32), amp=rng.uniform(0, 0.5, n + 1),
               period=rng.uniform(0.5, 2, n + 1), offset=rng.uniform(0, 1, n + 1), c=np.ones(n + 1))
     out = []
-    for order, blocks in ((1, 0), (0, 256), (1, 256), (2, 256), (3, 256), (2, 0), (3, 0)):
+    for order, blocks, store in ((1, 0, 0), (0, 256, 0), (1, 256, 0), (2, 256, 0), (3, 256, 0), (2, 0, 0),
+                                 (3, 0, 0), (2, 0, 1), (2, 0, 2)):
         with T.Engine(n, prec, "ref" if kind == "ref" else "rk4", "mf") as e:
             e.set_option(T.OPT_XCD_ORDER, order)
             e.set_option(T.OPT_TILE_BLOCKS, blocks)
+            e.set_option(T.OPT_STORE, store)
             if kind == "staged-tv":
                 e.set_time_variant(tv)
             else:
