@@ -480,7 +480,6 @@ static TileArgs<V> tile_base(tasep_ctx *c) {
     a.pmass = c->pmass + (size_t)c->slot * kMaxBlocks;
     a.n = c->n;
     a.ntiles = c->dim >> TileGeom<V>::b;
-    a.nt_out = nt_out(c);
     return a;
 }
 
@@ -527,17 +526,22 @@ static TileLaunch tile_launch(const tasep_ctx *c, const void *kernel, int &slots
     return L;
 }
 
-template <class V, int EPI, bool SELF = false>
-static void launch_tile(tasep_ctx *c, TileArgs<V> a, const char *name, double bytes) {
+template <class V, int EPI, bool SELF, bool NT>
+static uint32_t launch_tile_nt(tasep_ctx *c, TileArgs<V> a) {
     static int slots = 0;
-    const TileLaunch L = tile_launch(c, (const void *)k_tile_pass<V, EPI, SELF>, slots, a.ntiles);
-    const uint64_t g = L.grid;
+    const TileLaunch L = tile_launch(c, (const void *)k_tile_pass<V, EPI, SELF, NT>, slots, a.ntiles);
     a.xcd_order = L.order;
     a.xcd_res = L.res;
+    hipLaunchKernelGGL((k_tile_pass<V, EPI, SELF, NT>), dim3(L.grid), dim3(kTileThreads), 0, c->stream, a);
+    return (uint32_t)L.grid;
+}
+
+template <class V, int EPI, bool SELF = false>
+static void launch_tile(tasep_ctx *c, TileArgs<V> a, const char *name, double bytes) {
     prof_begin(c);
-    hipLaunchKernelGGL((k_tile_pass<V, EPI, SELF>), dim3(g), dim3(kTileThreads), 0, c->stream, a);
+    const uint32_t g = nt_out(c) ? launch_tile_nt<V, EPI, SELF, true>(c, a) : launch_tile_nt<V, EPI, SELF, false>(c, a);
     prof_end(c, name, bytes);
-    if (EPI == EPI_HORNER || EPI == EPI_RK4ACC) launch_finish(c, (uint32_t)g);
+    if (EPI == EPI_HORNER || EPI == EPI_RK4ACC) launch_finish(c, g);
 }
 
 template <class V>
@@ -615,14 +619,16 @@ static void ref_step(tasep_ctx *c, double h, uint64_t j) {
         t.h = h;
         t.n = c->n;
         t.ntiles = c->dim >> TileGeom<double>::b;
-        t.nt_out = nt_out(c);
-        static int slots = 0;
-        const TileLaunch L = tile_launch(c, (const void *)k_tile_ref, slots, t.ntiles);
+        static int slots[2] = {0, 0};
+        const bool nt = nt_out(c);
+        const void *kernel = nt ? (const void *)k_tile_ref<true> : (const void *)k_tile_ref<false>;
+        const TileLaunch L = tile_launch(c, kernel, slots[nt], t.ntiles);
         const uint64_t g = L.grid;
         t.xcd_order = L.order;
         t.xcd_res = L.res;
         prof_begin(c);
-        hipLaunchKernelGGL(k_tile_ref, dim3(g), dim3(kTileThreads), 0, c->stream, t);
+        if (nt) hipLaunchKernelGGL(k_tile_ref<true>, dim3(g), dim3(kTileThreads), 0, c->stream, t);
+        else hipLaunchKernelGGL(k_tile_ref<false>, dim3(g), dim3(kTileThreads), 0, c->stream, t);
         prof_end(c, "ref_tile", 16.0);
         launch_finish(c, (uint32_t)g);
         return;

@@ -89,9 +89,9 @@ __device__ __forceinline__ void st16(double *__restrict__ p, const double (&o)[2
 // Streaming (non-temporal) variants for vectors touched once per pass: the
 // self-term read of x and the output write.  Keeping them out of the way in
 // L2 leaves its capacity to the gathered vector, which neighbouring tiles
-// re-read.  The output's policy is chosen per launch (nt_out): a vector
-// that the memory-side cache can keep until the next pass reads it is
-// written with ordinary stores.
+// re-read.  The output's policy is a template parameter (NT) chosen per
+// launch: a vector that the memory-side cache can keep until the next pass
+// reads it is written with ordinary stores.
 #ifndef TASEP_NT_SELF
 #define TASEP_NT_SELF 1
 #endif
@@ -118,9 +118,9 @@ __device__ __forceinline__ void ld_self(const V *p, V (&o)[E]) {
     if constexpr (TASEP_NT_SELF) ld16_nt(p, o);
     else ld16(p, o);
 }
-template <class V, int E>
-__device__ __forceinline__ void st_out(V *p, const V (&o)[E], int nt) {
-    if (nt) st16_nt(p, o);
+template <bool NT, class V, int E>
+__device__ __forceinline__ void st_out(V *p, const V (&o)[E]) {
+    if constexpr (NT) st16_nt(p, o);
     else st16(p, o);
 }
 
@@ -145,7 +145,6 @@ struct TileArgs {
     uint64_t ntiles;
     int xcd_order;     // tile order (tile_walk)
     uint32_t xcd_res;  // workgroups resident per XCD (order 2)
-    int nt_out;        // outputs written non-temporally
 };
 
 
@@ -215,7 +214,7 @@ __device__ __forceinline__ int pop_bond(uint64_t &m, int LB0) {
     return k;
 }
 
-template <class V, int EPI, bool SELF>
+template <class V, int EPI, bool SELF, bool NT>
 __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(const TileArgs<V> a) {
     using G = TileGeom<V>;
     constexpr int E = G::E, e = G::e, LB = G::LB, C = G::C, TB0 = G::TB0;
@@ -447,7 +446,7 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
                     o[j] = fma(a.cy, kk, self);
                     ao[j] = a.stage > 0 ? (V)(ac[c][j] + (V)2 * kk) : kk;
                 }
-                st_out<V, E>(a.acc_out + wb + lo + ((uint32_t)c << LB), ao, a.nt_out);
+                st_out<NT>(a.acc_out + wb + lo + ((uint32_t)c << LB), ao);
             } else {
                 V dl[E];
 #pragma unroll
@@ -466,7 +465,7 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
                 chk.add(cs);
                 mass.add(ms);
             }
-            st_out<V, E>(a.out + wb + lo + ((uint32_t)c << LB), o, a.nt_out);
+            st_out<NT>(a.out + wb + lo + ((uint32_t)c << LB), o);
         }
         if (more) cur = gn;
     }
@@ -504,9 +503,9 @@ struct TileRefArgs {
     uint64_t ntiles;
     int xcd_order;
     uint32_t xcd_res;
-    int nt_out;         // next state written non-temporally
 };
 
+template <bool NT>
 __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) {
     using G = TileGeom<double>;
     constexpr int E = G::E, LB = G::LB, C = G::C, TB0 = G::TB0;
@@ -749,7 +748,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
             }
             chk.add(cs);
             mass.add(ms);
-            st_out<double, E>(a.out + wb + lo + ((uint32_t)c << LB), o, a.nt_out);
+            st_out<NT>(a.out + wb + lo + ((uint32_t)c << LB), o);
         }
         if (more) {
 #pragma unroll
