@@ -137,11 +137,17 @@ int tasep_moments(tasep_ctx *ctx, double *y, double *cov, double *mul,
 /* Window marginals of the current state in one device pass with FP64 sums:
  *   W[o*16 + p] = sum_s x_s [((s >> o) & 15) == p],  o = 0..n-4, p = 0..15
  * ((n-3)*16 doubles; n >= 4; FP32 contexts divided by the FP64 mass).
- * Every high-order output of the reference -- key_fit/high_order_corr,
- * error guess, closures, point functions (src/masterEquation.c:32-143,
- * :1422-1539), each a loop over all 2^n states per printed value -- is a
- * sum of these pattern probabilities over windows of <= 4 sites. */
+ * Every high-order output the reference writes for the final state --
+ * key_fit/high_order_corr, error guess, closures, point functions
+ * (src/masterEquation.c:32-143, :1422-1514), each a loop over all 2^n
+ * states per printed value -- is a sum of these pattern probabilities over
+ * windows of <= 4 sites. */
 int tasep_window_marginals(tasep_ctx *ctx, double *W);
+/* The same for windows of `width` = 4 or 6 sites: W[o*2^width + p],
+ * o = 0..n-width ((n-width+1)*2^width doubles; n >= width).  Width 6 covers
+ * the timed correlation of `-a` runs (high_order_corr with a 4-site key,
+ * src/masterEquation.c:1516-1537), whose sums span 6 sites. */
+int tasep_window_marginals_w(tasep_ctx *ctx, int width, double *W);
 
 /* Per-launch timing of one step, for measurement: runs `reps` steps from t0
  * (advancing the state like tasep_run with patience < 0), timing every

@@ -88,6 +88,7 @@ def lib():
         L.tor_key_sum.restype = C.c_double
         L.tor_key_sum.argtypes = [C.c_int, _f64p, C.c_uint64, C.c_int, C.c_int]
         L.tor_windows.argtypes = [C.c_int, _f64p, _f64p]
+        L.tor_windows_w.argtypes = [C.c_int, C.c_int, _f64p, _f64p]
         L.tor_dense.argtypes = [C.c_int, _f64p, _f64p]
         L.tor_ref_update4.restype = C.c_double
         L.tor_ref_update4.argtypes = [C.c_uint64, _f64p, _f64p, _f64p, _f64p, _f64p]
@@ -229,11 +230,15 @@ def key_sum(n, x, key, off, size):
     return float(lib().tor_key_sum(n, _f64(x), key, off, size))
 
 
-def windows(n, x):
-    """W[o, p] = sum_s x_s [((s >> o) & 15) == p], o = 0..n-4."""
-    W = np.zeros((n - 3) * 16, np.float64)
-    lib().tor_windows(n, _f64(x), W)
-    return W.reshape(n - 3, 16)
+def windows(n, x, width=4):
+    """W[o, p] = sum_s x_s [((s >> o) & (2^width - 1)) == p], o = 0..n-width."""
+    bins = 1 << width
+    W = np.zeros((n - width + 1) * bins, np.float64)
+    if width == 4:
+        lib().tor_windows(n, _f64(x), W)
+    else:
+        lib().tor_windows_w(n, width, _f64(x), W)
+    return W.reshape(n - width + 1, bins)
 
 
 def mass(x):

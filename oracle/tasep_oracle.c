@@ -341,15 +341,22 @@ double tor_key_sum(int n, const double *x, uint64_t key, int off, int size)
     return acc;
 }
 
-/* All 4-site window marginals: W[o*16 + p] = sum_s x_s [((s >> o) & 15) == p]
- * for o = 0..n-4 (no reference counterpart: the reference recomputes each
+/* All window marginals of `width` consecutive sites:
+ * W[o*2^width + p] = sum_s x_s [((s >> o) & (2^width - 1)) == p] for
+ * o = 0..n-width (no reference counterpart: the reference recomputes each
  * key_fit sum over all states; each of its sums is a sum of W entries). */
-void tor_windows(int n, const double *x, double *W)
+void tor_windows_w(int n, int width, const double *x, double *W)
 {
     uint64_t dim = (uint64_t)1 << n;
-    for (int q = 0; q < (n - 3) * 16; q++) W[q] = 0.;
+    const int bins = 1 << width;
+    for (int q = 0; q < (n - width + 1) * bins; q++) W[q] = 0.;
     for (uint64_t s = 0; s < dim; s++)
-        for (int o = 0; o + 4 <= n; o++) W[o * 16 + (int)((s >> o) & 15)] += x[s];
+        for (int o = 0; o + width <= n; o++) W[o * bins + (int)((s >> o) & (uint64_t)(bins - 1))] += x[s];
+}
+
+void tor_windows(int n, const double *x, double *W)
+{
+    tor_windows_w(n, 4, x, W);
 }
 
 double tor_mass(uint64_t dim, const double *x)

@@ -103,9 +103,14 @@ void tor_site_moments(int n, const double *x, double *y);
  * y from tor_site_moments. */
 void tor_cov(int n, const double *x, const double *y, double *cov, double *mul);
 
-/* Sequential sum of x (src/masterEquation.c:1009-1011). */
+/* key_fit sum over all states (src/masterEquation.c:32-45): sum of x_s
+ * over the states whose bits off..off+size-1 equal key. */
 double tor_key_sum(int n, const double *x, uint64_t key, int off, int size);
+/* Window marginals of `width` consecutive sites (4 for tor_windows):
+ * W[o*2^width + p] = sum_s x_s [((s >> o) & (2^width - 1)) == p]. */
 void tor_windows(int n, const double *x, double *W);
+void tor_windows_w(int n, int width, const double *x, double *W);
+/* Sequential sum of x (src/masterEquation.c:1009-1011). */
 double tor_mass(uint64_t dim, const double *x);
 
 /* The reference's update with four stage arrays, as written at

@@ -35,7 +35,7 @@ EXPORTS = [
     "tasep_batch_set_time_variant", "tasep_batch_set_x0_uniform", "tasep_batch_set_x",
     "tasep_batch_get_x", "tasep_batch_run", "tasep_batch_moments",
     "tasep_batch_set_option", "tasep_batch_get_stream", "tasep_window_marginals",
-    "tasep_batch_window_marginals", "tasep_batch_get_option",
+    "tasep_batch_window_marginals", "tasep_batch_get_option", "tasep_window_marginals_w",
 ]
 OPT_TILED, OPT_XCD_ORDER, OPT_RK4_STAGED, OPT_TILE_BLOCKS, OPT_LDS, OPT_LDS_THREADS = 1, 2, 3, 4, 5, 6
 OPT_GRAPH, OPT_STORE = 7, 8
@@ -126,6 +126,7 @@ def load_library():
     L.tasep_batch_get_option.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.tasep_batch_get_stream.argtypes = [vp, C.POINTER(vp)]
     L.tasep_window_marginals.argtypes = [vp, _f64p]
+    L.tasep_window_marginals_w.argtypes = [vp, C.c_int, _f64p]
     L.tasep_batch_window_marginals.argtypes = [vp, C.c_int, _f64p]
     for name in EXPORTS:
         if name not in ("tasep_strerror", "tasep_csr_nnz"):
@@ -253,11 +254,16 @@ class Engine:
                                             mul.ctypes.data, m.ctypes.data))
         return y, cov.reshape(n, n), mul.reshape(n, n), float(m[0])
 
-    def window_marginals(self):
-        """W[o, p] = sum_s x_s [((s >> o) & 15) == p], o = 0..n-4."""
-        W = np.zeros((self.n - 3) * 16)
-        _check(load_library().tasep_window_marginals(self._h, W))
-        return W.reshape(self.n - 3, 16)
+    def window_marginals(self, width=4):
+        """W[o, p] = sum_s x_s [((s >> o) & (2^width - 1)) == p],
+        o = 0..n-width, width 4 or 6."""
+        bins = 1 << width
+        W = np.zeros((self.n - width + 1) * bins)
+        if width == 4:
+            _check(load_library().tasep_window_marginals(self._h, W))
+        else:
+            _check(load_library().tasep_window_marginals_w(self._h, width, W))
+        return W.reshape(self.n - width + 1, bins)
 
     def step_profile(self, t0, h, reps=5):
         """[(name, mean ms, algorithmic bytes/state)] for each launch of a step."""

@@ -63,6 +63,7 @@ int device_moments(hipStream_t stream, const void *x, int n, int prec, double **
 
 // Window marginals W[o*16 + p] (o = 0..n-4) of a device state vector, as
 // tasep_window_marginals.
-int device_windows(hipStream_t stream, const void *x, int n, int prec, double **part, double **out, double *W);
+int device_windows(hipStream_t stream, const void *x, int n, int prec, double **part, double **out, int width,
+                   double *W);
 
 }  // namespace tasep_host

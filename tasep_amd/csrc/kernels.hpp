@@ -481,34 +481,39 @@ __global__ __launch_bounds__(kBlock) void k_moments_partial(const V *__restrict_
 // ===========================================================================
 // Window marginals (pattern probabilities behind the reference's high-order
 // post-processing, src/masterEquation.c:32-143, :1422-1539, which loops over
-// all 2^n states for every value it prints):
-//   W[o][p] = sum_s x_s [((s >> o) & 15) == p],  o = 0..n-4, p = 0..15,
+// all 2^n states for every value it prints), for windows of WW = 4 or 6
+// consecutive sites (B = 2^WW patterns):
+//   W[o][p] = sum_s x_s [((s >> o) & (B-1)) == p],  o = 0..n-WW, p < B,
 // plus the mass as the last quantity.  Rows of 1024 states go through LDS;
-// lane o*16+p (o < 10) owns bin (o, p) and sums its 64 row states (windows
-// inside bits 0-9) or, for windows that reach into the row bits, the
-// 2^o states of the matching low pattern when the row's bits match; lane
-// 192+j owns the 16 bins of offset 10+j, whose pattern is fixed per row, and
-// adds the row mass to one of them.  Sums run in a fixed order per block and
-// k_moments_final adds the block partials in block order: deterministic.
+// lane t owns the bins o*B+p = t, t+256, ... with o < 10 and sums its row
+// states (windows inside bits 0-9) or, for windows that reach into the row
+// bits, the 2^o states of the matching low pattern when the row's bits
+// match; lane j < n-WW+1-10 also owns the B bins of offset 10+j, whose
+// pattern is fixed per row, and adds the row mass to one of them.  Sums run
+// in a fixed order per block and k_moments_final adds the block partials in
+// block order: deterministic.
 // ===========================================================================
-constexpr int kWin = 4;
+constexpr int kWin = 4, kWin6 = 6;
 constexpr int kWinBins = 1 << kWin;
+// Quantities of the widest window table: (kMaxN - 5) * 64 + 1.
+constexpr int kWinMaxQ = (kMaxN - kWin6 + 1) * (1 << kWin6) + 1;
 
-template <class V>
+template <class V, int WW>
 __global__ __launch_bounds__(kBlock) void k_windows_partial(const V *__restrict__ x, int n, uint64_t dim,
                                                             uint64_t rows, double *__restrict__ part) {
     constexpr int R = kMomRowBits;
+    constexpr int B = 1 << WW;
+    constexpr int QL = (R * B + kBlock - 1) / kBlock;    // low bins per lane
     __shared__ double zs[1 << R];
-    __shared__ double hi_acc[kMaxN * kWinBins];
+    __shared__ double hi_acc[kMaxN * B];
     __shared__ double red[kWaves];
     const int t = threadIdx.x;
-    const int no = n - kWin + 1;                       // window offsets
-    const int nq = no * kWinBins + 1;
-    const int nlow = (no < R ? no : R) * kWinBins;      // bins with o < R
-    const int nhigh = no > R ? no - R : 0;              // offsets >= R
-    for (int i = t; i < kMaxN * kWinBins; i += kBlock) hi_acc[i] = 0.0;
-    const int o = t / kWinBins, p = t % kWinBins;
-    Kahan acc, macc;
+    const int no = n - WW + 1;                           // window offsets
+    const int nq = no * B + 1;
+    const int nlow = (no < R ? no : R) * B;              // bins with o < R
+    const int nhigh = no > R ? no - R : 0;               // offsets >= R
+    for (int i = t; i < kMaxN * B; i += kBlock) hi_acc[i] = 0.0;
+    Kahan acc[QL], macc;
     for (uint64_t row = blockIdx.x; row < rows; row += gridDim.x) {
         const uint64_t base = row << R;
         double v[4];
@@ -520,32 +525,36 @@ __global__ __launch_bounds__(kBlock) void k_windows_partial(const V *__restrict_
         }
         const double rm = block_sum((v[0] + v[1]) + (v[2] + v[3]), red);  // also orders zs
         macc.add(rm);
-        if (t < nlow) {
+#pragma unroll
+        for (int k = 0; k < QL; k++) {
+            const int q = t + k * kBlock;
+            if (q >= nlow) break;
+            const int o = q / B, p = q % B;
             double sum = 0.0;
-            if (o + kWin <= R) {
-                for (int i = 0; i < (1 << (R - kWin)); i++) {
+            if (o + WW <= R) {
+                for (int i = 0; i < (1 << (R - WW)); i++) {
                     const int lo = i & ((1 << o) - 1), hi = i >> o;
-                    sum += zs[(hi << (o + kWin)) | (p << o) | lo];
+                    sum += zs[(hi << (o + WW)) | (p << o) | lo];
                 }
             } else {
                 const int lowbits = R - o;                 // window bits inside the row
-                const uint64_t hmask = ((uint64_t)1 << (kWin - lowbits)) - 1;
+                const uint64_t hmask = ((uint64_t)1 << (WW - lowbits)) - 1;
                 if ((row & hmask) == (uint64_t)(p >> lowbits)) {
                     const int pl = p & ((1 << lowbits) - 1);
                     for (int lo = 0; lo < (1 << o); lo++) sum += zs[(pl << o) | lo];
                 }
             }
-            acc.add(sum);
-        } else if (t >= 192 && t < 192 + nhigh) {
-            const int j = t - 192;
-            hi_acc[j * kWinBins + (int)((row >> j) & (kWinBins - 1))] += rm;
+            acc[k].add(sum);
         }
+        if (t < nhigh) hi_acc[t * B + (int)((row >> t) & (B - 1))] += rm;
         __syncthreads();  // zs is rewritten by the next row
     }
     double *out = part + (uint64_t)blockIdx.x * nq;
-    if (t < nlow) out[t] = acc.value();
+#pragma unroll
+    for (int k = 0; k < QL; k++)
+        if (t + k * kBlock < nlow) out[t + k * kBlock] = acc[k].value();
     __syncthreads();
-    for (int q = t; q < nhigh * kWinBins; q += kBlock) out[R * kWinBins + q] = hi_acc[q];
+    for (int q = t; q < nhigh * B; q += kBlock) out[R * B + q] = hi_acc[q];
     if (t == 0) out[nq - 1] = macc.value();
 }
 

@@ -481,7 +481,7 @@ int tasep_batch_window_marginals(tasep_batch *b, int i, double *W) {
     if (!W) return fail(TASEP_ERR_ARG, "NULL argument");
     if (b->n[i] < 4) return fail(TASEP_ERR_ARG, "window marginals need n >= 4");
     const void *x = (const char *)b->x + b->off[i] * b->esize;
-    return device_windows(b->stream, x, b->n[i], b->prec, &b->mom_part, &b->mom_out, W);
+    return device_windows(b->stream, x, b->n[i], b->prec, &b->mom_part, &b->mom_out, 4, W);
 }
 
 int tasep_batch_get_option(tasep_batch *b, int key, int64_t *value) {

@@ -998,7 +998,7 @@ int tasep_host::device_moments(hipStream_t stream, const void *x, int n, int pre
     const int nb = (int)(rows < (uint64_t)kMomMaxBlocks ? rows : kMomMaxBlocks);
     if (!*part) {
         HIP_OK(hipMalloc(part, (size_t)kMomMaxBlocks * kMomMaxQ * sizeof(double)));
-        HIP_OK(hipMalloc(out, kMomMaxQ * sizeof(double)));
+        HIP_OK(hipMalloc(out, kWinMaxQ * sizeof(double)));
     }
     if (prec == TASEP_F32)
         hipLaunchKernelGGL(k_moments_partial<float>, dim3(nb), dim3(kBlock), 0, stream, (const float *)x, n, dim,
@@ -1032,24 +1032,35 @@ int tasep_host::device_moments(hipStream_t stream, const void *x, int n, int pre
     return TASEP_OK;
 }
 
-// Window marginals of a state vector (2^n entries, n >= 4) on the device:
-// W[o*16 + p] for o = 0..n-4; FP32 vectors are divided by their FP64 mass.
+template <int WW>
+static void launch_windows(hipStream_t stream, const void *x, int n, int prec, uint64_t dim, uint64_t rows, int nb,
+                           double *part) {
+    if (prec == TASEP_F32)
+        hipLaunchKernelGGL((k_windows_partial<float, WW>), dim3(nb), dim3(kBlock), 0, stream, (const float *)x, n,
+                           dim, rows, part);
+    else
+        hipLaunchKernelGGL((k_windows_partial<double, WW>), dim3(nb), dim3(kBlock), 0, stream, (const double *)x,
+                           n, dim, rows, part);
+}
+
+// Window marginals of a state vector (2^n entries, n >= width) on the
+// device: W[o*2^width + p] for o = 0..n-width, width 4 or 6; FP32 vectors are
+// divided by their FP64 mass.  The block count is capped so that the
+// partials fit the moment buffer (kMomMaxBlocks x kMomMaxQ doubles).
 int tasep_host::device_windows(hipStream_t stream, const void *x, int n, int prec, double **part, double **out,
-                               double *W) {
+                               int width, double *W) {
     const uint64_t dim = (uint64_t)1 << n;
-    const int nq = (n - kWin + 1) * kWinBins + 1;
+    const int nq = (n - width + 1) * (1 << width) + 1;
     const uint64_t rows = dim >> kMomRowBits ? dim >> kMomRowBits : 1;
-    const int nb = (int)(rows < (uint64_t)kMomMaxBlocks ? rows : kMomMaxBlocks);
+    const uint64_t cap = (uint64_t)kMomMaxBlocks * kMomMaxQ / (uint64_t)nq;
+    const uint64_t maxb = cap < (uint64_t)kMomMaxBlocks ? cap : (uint64_t)kMomMaxBlocks;
+    const int nb = (int)(rows < maxb ? rows : maxb);
     if (!*part) {
         HIP_OK(hipMalloc(part, (size_t)kMomMaxBlocks * kMomMaxQ * sizeof(double)));
-        HIP_OK(hipMalloc(out, kMomMaxQ * sizeof(double)));
+        HIP_OK(hipMalloc(out, kWinMaxQ * sizeof(double)));
     }
-    if (prec == TASEP_F32)
-        hipLaunchKernelGGL(k_windows_partial<float>, dim3(nb), dim3(kBlock), 0, stream, (const float *)x, n, dim,
-                           rows, *part);
-    else
-        hipLaunchKernelGGL(k_windows_partial<double>, dim3(nb), dim3(kBlock), 0, stream, (const double *)x, n,
-                           dim, rows, *part);
+    if (width == kWin6) launch_windows<kWin6>(stream, x, n, prec, dim, rows, nb, *part);
+    else launch_windows<kWin>(stream, x, n, prec, dim, rows, nb, *part);
     hipLaunchKernelGGL(k_moments_final, dim3(nq), dim3(kBlock), 0, stream, *part, nb, nq, *out);
     HIP_OK(hipGetLastError());
     std::vector<double> E(nq);
@@ -1061,7 +1072,12 @@ int tasep_host::device_windows(hipStream_t stream, const void *x, int n, int pre
 }
 
 extern "C" int tasep_window_marginals(tasep_ctx *c, double *W) {
+    return tasep_window_marginals_w(c, kWin, W);
+}
+
+extern "C" int tasep_window_marginals_w(tasep_ctx *c, int width, double *W) {
     if (!c || !W) return fail(TASEP_ERR_ARG, "NULL argument");
-    if (c->n < kWin) return fail(TASEP_ERR_ARG, "window marginals need n >= 4");
-    return device_windows(c->stream, xcur(c), c->n, c->prec, &c->mom_part, &c->mom_out, W);
+    if (width != kWin && width != kWin6) return fail(TASEP_ERR_ARG, "window width: 4 or 6");
+    if (c->n < width) return fail(TASEP_ERR_ARG, "window marginals need n >= width");
+    return device_windows(c->stream, xcur(c), c->n, c->prec, &c->mom_part, &c->mom_out, width, W);
 }

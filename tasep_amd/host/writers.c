@@ -25,20 +25,23 @@
 #include <omp.h>
 
 /* ------------------------------------------------------------ snapshots */
+/* Snapshots keep the moments and the 6-site window marginals (all the
+ * timed correlation file needs); the whole state only for the snapshot
+ * probabilities file. */
 void snapshots_init(snapshots_t *s, int n, uint64_t dim, int res, unsigned int steps, int keep_x)
 {
     memset(s, 0, sizeof *s);
     s->n = n;
     s->dim = dim;
-    (void)keep_x;
     if (res <= 0) return;
     s->enabled = 1;
     s->cap = (uint64_t)steps / (uint64_t)res + 2;
     s->y = (double *)malloc(s->cap * (size_t)n * sizeof(double));
     s->cov = (double *)malloc(s->cap * (size_t)n * n * sizeof(double));
     s->mul = (double *)malloc(s->cap * (size_t)n * n * sizeof(double));
-    s->x = (double *)malloc(s->cap * dim * sizeof(double));
-    if (!s->y || !s->cov || !s->mul || !s->x) {
+    if (keep_x) s->x = (double *)malloc(s->cap * dim * sizeof(double));
+    if (n >= 6) s->w6 = (double *)malloc(s->cap * (size_t)(n - 5) * 64 * sizeof(double));
+    if (!s->y || !s->cov || !s->mul || (keep_x && !s->x) || (n >= 6 && !s->w6)) {
         printf("ERROR: Not enough memory to allocate animation data array\n");
         exit(-1);
     }
@@ -50,7 +53,8 @@ void snapshots_take(snapshots_t *s, tasep_ctx *ctx)
     const int n = s->n;
     const uint64_t c = s->count;
     int rc = tasep_moments(ctx, s->y + c * n, s->cov + c * n * n, s->mul + c * n * n, NULL);
-    if (!rc) rc = tasep_get_x(ctx, s->x + c * s->dim);
+    if (!rc && s->x) rc = tasep_get_x(ctx, s->x + c * s->dim);
+    if (!rc && s->w6) rc = tasep_window_marginals_w(ctx, 6, s->w6 + c * (uint64_t)(n - 5) * 64);
     if (rc) {
         fprintf(stderr, "ERROR: snapshot: %s\n", tasep_strerror(rc));
         exit(-1);
@@ -64,6 +68,7 @@ void snapshots_free(snapshots_t *s)
     free(s->cov);
     free(s->mul);
     free(s->x);
+    free(s->w6);
     memset(s, 0, sizeof *s);
 }
 
@@ -183,70 +188,57 @@ static void write_anim_dir(const out_ctx_t *oc, const char *suffix, const px_t *
 }
 
 /* ------------------------------------------------- high-order functions */
-/* bits [off, off+size) of s equal the low `size` bits of key
- * (key_fit, src/masterEquation.c:32-45) */
-static int key_match(uint64_t s, uint64_t key, int off, int size)
-{
-    const uint64_t m = size >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << size) - 1);
-    return ((s >> off) & m) == (key & m);
-}
-
 typedef struct {
     double lr, no, left, right;
 } keysums_t;
 
-/* the four pattern marginals of src/masterEquation.c:61-68 */
-static keysums_t key_sums(const double *x, uint64_t dim, uint64_t key, int ks, int off, int with_lr)
+/* key_fit sum (src/masterEquation.c:32-45) over states whose bits
+ * off..off+size-1 equal key, from the window marginals
+ * W[o*2^width + p] = P(bits o..o+width-1 = p) (size <= width) */
+static double win_sum_w(const double *W, int n, int width, uint64_t key, int off, int size)
 {
-    const uint64_t k_left = key + ((uint64_t)1 << ks);
-    const uint64_t k_right = (key << 1) + 1;
-    const uint64_t k_lr = (key << 1) + 1 + ((uint64_t)1 << (ks + 1));
-    keysums_t r = {0, 0, 0, 0};
-    for (uint64_t s = 0; s < dim; s++) {
-        if (with_lr) r.lr += key_match(s, k_lr, off - 1, ks + 2) ? x[s] : 0.;
-        r.no += key_match(s, key, off, ks) ? x[s] : 0.;
-        r.left += key_match(s, k_left, off, ks + 1) ? x[s] : 0.;
-        r.right += key_match(s, k_right, off - 1, ks + 1) ? x[s] : 0.;
-    }
-    return r;
-}
-
-static double high_order_corr(const double *x, uint64_t dim, uint64_t key, int ks, int off)
-{
-    const keysums_t k = key_sums(x, dim, key, ks, off, 1);
-    const double ret = k.lr * k.no - k.left * k.right;
-    return ret / sqrt(k.left * (k.no - k.left) * k.right * (k.no - k.right));
-}
-
-/* key_fit sum over states whose bits off..off+size-1 equal key (size <= 4),
- * from the window marginals W[o*16 + p] = P(bits o..o+3 = p) */
-static double win_sum(const double *W, int n, uint64_t key, int off, int size)
-{
-    const int w = off < n - 4 ? off : n - 4;
+    const int w = off < n - width ? off : n - width;
     const int sh = off - w;
-    const unsigned m = (1u << size) - 1;
+    const unsigned m = (1u << size) - 1, bins = 1u << width;
     double acc = 0;
-    for (unsigned p = 0; p < 16; p++)
-        if (((p >> sh) & m) == (unsigned)key) acc += W[w * 16 + (int)p];
+    for (unsigned p = 0; p < bins; p++)
+        if (((p >> sh) & m) == (unsigned)key) acc += W[(size_t)w * bins + p];
     return acc;
 }
 
-/* the four pattern marginals of src/masterEquation.c:61-68 from W */
-static keysums_t win_sums(const double *W, int n, uint64_t key, int ks, int off)
+static double win_sum(const double *W, int n, uint64_t key, int off, int size)
+{
+    return win_sum_w(W, n, 4, key, off, size);
+}
+
+/* the four pattern marginals of src/masterEquation.c:61-68 from W; the
+ * widest one spans ks + 2 sites */
+static keysums_t win_sums_w(const double *W, int n, int width, uint64_t key, int ks, int off)
 {
     keysums_t r;
-    r.lr = win_sum(W, n, (key << 1) + 1 + ((uint64_t)1 << (ks + 1)), off - 1, ks + 2);
-    r.no = win_sum(W, n, key, off, ks);
-    r.left = win_sum(W, n, key + ((uint64_t)1 << ks), off, ks + 1);
-    r.right = win_sum(W, n, (key << 1) + 1, off - 1, ks + 1);
+    r.lr = win_sum_w(W, n, width, (key << 1) + 1 + ((uint64_t)1 << (ks + 1)), off - 1, ks + 2);
+    r.no = win_sum_w(W, n, width, key, off, ks);
+    r.left = win_sum_w(W, n, width, key + ((uint64_t)1 << ks), off, ks + 1);
+    r.right = win_sum_w(W, n, width, (key << 1) + 1, off - 1, ks + 1);
     return r;
+}
+
+static keysums_t win_sums(const double *W, int n, uint64_t key, int ks, int off)
+{
+    return win_sums_w(W, n, 4, key, ks, off);
+}
+
+/* high_order_corr (src/masterEquation.c:47-75) as the normalised
+ * correlation it computes */
+static double corr_of(keysums_t k)
+{
+    const double ret = k.lr * k.no - k.left * k.right;
+    return ret / sqrt(k.left * (k.no - k.left) * k.right * (k.no - k.right));
 }
 
 static double win_corr(const double *W, int n, uint64_t key, int ks, int off)
 {
-    const keysums_t k = win_sums(W, n, key, ks, off);
-    const double ret = k.lr * k.no - k.left * k.right;
-    return ret / sqrt(k.left * (k.no - k.left) * k.right * (k.no - k.right));
+    return corr_of(win_sums(W, n, key, ks, off));
 }
 
 static double error_guess(const double *W, int n, uint64_t key, int ks, int off)
@@ -477,8 +469,11 @@ void write_outputs(const out_ctx_t *oc, const double *x, const double *y, const 
         if ((f = fopen(name, "w"))) {
             for (uint64_t i = 0; i < snaps->count; i++) {
                 fprintf(f, "%f ", oc->h * oc->anim_res * (double)i);
+                /* 4-site key: sums over 6 sites, from the snapshot's
+                 * 6-site window marginals */
                 for (int k = 1; k < n - ms; k++)
-                    fprintf(f, "%1.7g ", high_order_corr(snaps->x + i * dim, dim, mask, ms, n - ms - k));
+                    fprintf(f, "%1.7g ", corr_of(win_sums_w(snaps->w6 + i * (uint64_t)(n - 5) * 64, n, 6, mask,
+                                                            ms, n - ms - k)));
                 fprintf(f, "\n");
             }
             fclose(f);

@@ -17,7 +17,8 @@ typedef struct {
     double *y;    /* count * n        */
     double *cov;  /* count * n * n    */
     double *mul;  /* count * n * n    */
-    double *x;    /* count * dim      */
+    double *x;    /* count * dim, only when the probabilities are written */
+    double *w6;   /* count * (n-5) * 64 six-site window marginals (n >= 6) */
 } snapshots_t;
 
 void snapshots_init(snapshots_t *s, int n, uint64_t dim, int res, unsigned int steps, int keep_x);

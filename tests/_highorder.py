@@ -10,13 +10,14 @@ does, so a test can compare the text with the reference's files.
 """
 
 
-def key_sum(W, n, key, off, size):
-    """sum_s x_s over states whose bits off..off+size-1 equal key."""
-    w = min(off, n - 4)
+def key_sum(W, n, key, off, size, width=4):
+    """sum_s x_s over states whose bits off..off+size-1 equal key, from
+    windows of `width` sites."""
+    w = min(off, n - width)
     sh = off - w
     m = (1 << size) - 1
     acc = 0.0
-    for p in range(16):
+    for p in range(1 << width):
         if ((p >> sh) & m) == key:
             acc += W[w][p]
     return acc

@@ -179,6 +179,26 @@ def test_window_marginals_vs_oracle(n):
     assert np.max(np.abs(W32 - oracle.windows(n, x))) <= 1e-7
 
 
+@pytest.mark.parametrize("n", [6, 9, 10, 11, 13, 16, 17, 20])
+def test_six_site_window_marginals_vs_oracle(n):
+    """6-site windows (row-internal, row-straddling and row-index offsets of
+    the device pass for n up to 20) against the oracle, and their 4-site
+    marginals against the device's 4-site table."""
+    x = np.random.default_rng(130 + n).random(1 << n)
+    x /= x.sum()
+    with T.Engine(n, "f64", "ref", "mf") as e:
+        e.set_x(x)
+        W6 = e.window_marginals(6)
+        W4 = e.window_marginals()
+    assert np.max(np.abs(W6 - oracle.windows(n, x, 6))) <= 1e-13
+    for o in range(n - 5):
+        assert np.max(np.abs(W6[o].reshape(4, 16).sum(axis=0) - W4[o])) <= 1e-13
+    with T.Engine(n, "f32", "rk4", "mf") as e:
+        e.set_x(2.0 * x)
+        W32 = e.window_marginals(6)
+    assert np.max(np.abs(W32 - oracle.windows(n, x, 6))) <= 1e-7
+
+
 # ------------------------------------------------------------------- RK4 mode
 @pytest.mark.parametrize("gen", ["mf", "csr"])
 def test_rk4_f64_vs_oracle(gen):

@@ -127,6 +127,44 @@ def test_host_snapshots_option(tmp_path):
 
 @needs_bin
 @pytest.mark.gpu
+def test_host_timed_correlation_vs_oracle_states(tmp_path):
+    """-a at n=8: the timed correlation file (4-site key, sums over 6 sites)
+    comes from the device's 6-site window marginals of each snapshot; its
+    values equal the correlation formula evaluated with the oracle's
+    state-loop key_fit sums on the oracle's states after steps 1, 11, 21, ...
+    (printed with 7 significant digits; the reference's own values are
+    undefined, so only the formula is checked)."""
+    import oracle
+    n, steps, a = 8, 100, 10
+    r = [0.4] + [1.0] * (n - 1) + [0.6]
+    (tmp_path / "rates.txt").write_text(" ".join("%g" % v for v in r))
+    p = _run(["-n", str(n), "-i", "rates.txt", "-t", "0", "-T", "1", "-s", str(steps), "-a", str(a),
+              "-o", "out"], tmp_path, stdin="Y\nY\n")
+    assert p.returncode == 0, p.stderr
+    rows = [ln.split() for ln in open(tmp_path / "out_highcorr_timed").read().splitlines()]
+    assert len(rows) == 10
+    h = oracle.step_h(1.0, 0.0, steps)
+    x = np.full(1 << n, 2.0 ** -n)
+    done = 0
+    for i, row in enumerate(rows):
+        todo = 1 if i == 0 else a
+        x, _ = oracle.run_ref(n, r, x, 0.0, h, todo, patience=-1)
+        done += todo
+        assert float(row[0]) == pytest.approx(h * a * i, abs=1e-6)
+        vals = [float(v) for v in row[1:]]
+        assert len(vals) == n - 5
+        for k, got in zip(range(1, n - 4), vals):
+            off = n - 4 - k
+            lr = oracle.key_sum(n, x, 63, off - 1, 6)
+            no = oracle.key_sum(n, x, 15, off, 4)
+            left = oracle.key_sum(n, x, 31, off, 5)
+            right = oracle.key_sum(n, x, 31, off - 1, 5)
+            want = (lr * no - left * right) / np.sqrt(left * (no - left) * right * (no - right))
+            assert abs(got - want) <= 1e-6 * abs(want) + 1e-12, (i, k, got, want)
+
+
+@needs_bin
+@pytest.mark.gpu
 @pytest.mark.parametrize("n,fast", [(14, False), (20, False), (20, True)])
 def test_host_large_lattice_vs_oracle(tmp_path, n, fast):
     """Larger lattices than the fixtures (other kernels: one lane per state,

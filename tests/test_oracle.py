@@ -263,6 +263,30 @@ def test_windows_equal_reference_key_sums():
                     assert abs(H.key_sum(W, n, key, off, size) - want) <= 1e-12 * x.sum()
 
 
+def test_six_site_windows_equal_key_sums():
+    """6-site window marginals (the timed correlation's sums span 6 sites):
+    every bin is the key_fit sum of its pattern, every key_fit sum of up to
+    6 sites is a sum of bins, and dropping two edge sites gives the 4-site
+    windows."""
+    from tests import _highorder as H
+    for n in (6, 7, 9):
+        x = np.random.default_rng(70 + n).random(1 << n)
+        W6 = oracle.windows(n, x, 6)
+        assert W6.shape == (n - 5, 64)
+        for o in range(n - 5):
+            for p in range(64):
+                assert abs(W6[o, p] - oracle.key_sum(n, x, p, o, 6)) <= 1e-13 * x.sum()
+        for size in (4, 5, 6):
+            for off in range(0, n - size + 1):
+                for key in range(1 << size):
+                    want = oracle.key_sum(n, x, key, off, size)
+                    assert abs(H.key_sum(W6, n, key, off, size, width=6) - want) <= 1e-12 * x.sum()
+        W4 = oracle.windows(n, x)
+        for o in range(n - 5):
+            assert np.allclose(W6[o].reshape(4, 16).sum(axis=0), W4[o], rtol=1e-13, atol=0)
+            assert np.allclose(W6[o].reshape(16, 4).sum(axis=1), W4[o + 2], rtol=1e-13, atol=0)
+
+
 def test_windows_invariants():
     n = 9
     x = np.random.default_rng(3).random(1 << n)
