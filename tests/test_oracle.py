@@ -7,7 +7,8 @@ Pins, in order of strength:
   * the reference's own functions compiled from source (oracle/_ref), where
     present;
   * mathematics: closed forms (nnz, row length), column sums of a generator,
-    exact stationary states, the F1 identity k1 = k2 = k3 = k4.
+    exact stationary states, the identity k1 = k2 = k3 = k4 of the
+    reference's stages.
 """
 import numpy as np
 import pytest
@@ -142,9 +143,10 @@ def test_moments_format_vs_reference_cli(name):
     assert G.read_text(c, "out_cov.mat") == mat
 
 
-def test_ref_stages_are_equal_F1():
+def test_ref_stages_are_equal():
     """All four reference stages read the unmodified x: with the reference's
-    own rk4_step, k1..k4 are bitwise equal (survey finding F1)."""
+    own rk4_step, k1..k4 are bitwise equal, so its "RK4" update is an Euler
+    step with the ((k + 2k) + 2k + k) / 6 rounding."""
     if not oracle.have_ref():
         pytest.skip("oracle/_ref not built here")
     R = oracle.ref_lib()
@@ -200,7 +202,8 @@ def test_rk4_oracle_matches_expm_at_fourth_order():
 
 
 def test_ref_loop_is_explicit_euler():
-    """F1: the reference update equals x + h L x (up to its 1-2-2-1 rounding)."""
+    """With k1 = k2 = k3 = k4 the reference update equals x + h L x (up to
+    its 1-2-2-1 rounding)."""
     n = 5
     r = np.random.default_rng(6).uniform(0.2, 1.5, n + 1)
     L = oracle.dense(n, r)
