@@ -307,12 +307,13 @@ def test_full_size_n28_bernoulli_product_is_stationary(kind):
     assert abs(mass - 1.0) <= 1e-5
 
 
-@pytest.mark.parametrize("n,prec", [(32, "f32"), (28, "f64")])
+@pytest.mark.parametrize("n,prec", [(32, "f32"), (28, "f64"), (34, "f32")])
 def test_full_size_uniform_is_stationary(n, prec):
     """alpha = beta = 1/2, gamma = 1: the uniform distribution is exactly
     stationary (every state's out-rate equals its in-rate), at the largest
-    benchmark size (n=32, 2^32 FP32 states, 16 GiB per vector) and at n=28
-    in FP64."""
+    benchmark size (n=32, 2^32 FP32 states, 16 GiB per vector), at n=28 in
+    FP64 and at the largest lattice a context takes (n=34, 64 GiB per FP32
+    vector, 192 GiB for the three Horner vectors)."""
     r = np.array([0.5] + [1.0] * (n - 1) + [0.5])
     with T.Engine(n, prec, "rk4", "mf") as e:
         e.set_rates(r)
