@@ -112,6 +112,23 @@ def test_ref_loop_csr_rows_across_matrix_chunks():
     assert _bits_equal(xs[0], xs[1])
 
 
+def test_ref_loop_csr_at_its_largest_lattice_n30():
+    """n=30, the largest lattice the stored matrix takes (9.9e9 nonzeros,
+    167 GB of CSR arrays with 64-bit indices): two reference steps through
+    the device-built matrix equal the matrix-free tiled loop bit for bit."""
+    n = 30
+    r = np.random.default_rng(30).uniform(0.1, 2.0, n + 1)
+    xs = []
+    for gen in ("csr", "mf"):
+        with T.Engine(n, "f64", "ref", gen) as e:
+            e.set_rates(r)
+            e.set_x0_uniform()
+            info = e.run(0.0, 0.01, 2, patience=-1)
+            assert info.steps_done == 2
+            xs.append(e.get_x())
+    assert _bits_equal(xs[0], xs[1])
+
+
 def test_ref_early_stop_n10_uniform_is_stationary():
     """n=10, alpha=beta=1/2, gamma=1 from the uniform state: the reference
     leaves after 101 sub-threshold steps (index 100, t = 1.01)."""
