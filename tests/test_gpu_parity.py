@@ -239,6 +239,32 @@ def test_rk4_f64_vs_oracle(gen):
     assert np.max(np.abs(x - ox)) <= 1e-12
 
 
+@pytest.mark.parametrize("n,tv", [(20, False), (20, True), (22, False)])
+def test_rk4_f64_tiles_vs_oracle(n, tv):
+    """RK4 on FP64 tiles (Horner passes for constant rates, stored stages for
+    time-variant ones) against the oracle's classical RK4 from a random
+    state, 12 steps, every state within 1e-12 of the largest probability
+    (the two sum in different orders; rounding stays near 1e-15)."""
+    rng = np.random.default_rng(200 + n)
+    r = rng.uniform(0.2, 1.5, n + 1)
+    tvd = dict(type=(np.arange(n + 1) % 3).astype(np.int32), amp=rng.uniform(0, 0.5, n + 1),
+               period=rng.uniform(0.5, 2, n + 1), offset=rng.uniform(0, 1, n + 1), c=np.ones(n + 1))
+    x0 = rng.random(1 << n)
+    x0 /= x0.sum()
+    steps = 12
+    ox, _ = oracle.run_rk4(n, None if tv else r, x0, 0.0, 0.01, steps, tv=tvd if tv else None)
+    with T.Engine(n, "f64", "rk4", "mf") as e:
+        if tv:
+            e.set_time_variant(tvd)
+        else:
+            e.set_rates(r)
+        e.set_x(x0)
+        info = e.run(0.0, 0.01, steps, patience=-1)
+        x = e.get_x()
+    assert info.steps_done == steps
+    assert np.max(np.abs(x - ox)) <= 1e-12 * np.max(np.abs(ox))
+
+
 def test_rk4_f64_time_variant_vs_oracle():
     n = 9
     rng = np.random.default_rng(4)
