@@ -67,6 +67,25 @@ def test_batch_matches_oracle_per_instance(prec, tol, schedule):
         assert abs(infos[i].mass - 1.0) <= (1e-12 if prec == "f64" else 1e-5)
 
 
+@pytest.mark.parametrize("schedule", [1, 2])
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
+def test_batch_large_instances_match_oracle(prec, tol, schedule):
+    """Instances larger than the benchmark batch's (n = 18, 20, 22: up to
+    12 tile-index bits in the tile schedule) beside small ones, each against
+    its own oracle run."""
+    steps = 12
+    b, spec = _setup([20, 9, 22, 18], 77, prec, schedule=schedule)
+    with b:
+        infos = b.run(0.0, 0.01, steps)
+        xs = [b.get_x(i) for i in range(b.count)]
+        assert b.get_option(T.BATCH_OPT_SCHEDULE) == schedule
+    for i, n in enumerate(b.ns):
+        ox = _oracle(n, spec[i], steps)
+        err = np.max(np.abs(xs[i] - ox))
+        assert err <= tol, (i, n, spec[i][0], err)
+        assert infos[i].steps_done == steps
+
+
 def test_batch_lds_instances_equal_single_engine_bitwise():
     """LDS-resident batch instances run the same kernel as a single context
     (RK4 keeps four vectors in LDS: n <= 12 in FP64, n <= 13 in FP32)."""
