@@ -165,6 +165,32 @@ def test_host_timed_correlation_vs_oracle_states(tmp_path):
 
 @needs_bin
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_host_rk4_environment_knobs(tmp_path, prec):
+    """TASEP_INTEGRATOR=rk4 (and TASEP_PRECISION=f32) select classical RK4 in
+    the host without new options: the printed densities (%lf, entry site
+    first) equal the oracle's RK4 densities to the printed digits, and the
+    probabilities file to 1e-12 (FP64) / 1e-9 (FP32)."""
+    import oracle
+    n, steps, T = 12, 200, 2.0
+    r = np.random.default_rng(12).uniform(0.2, 1.5, n + 1)
+    (tmp_path / "rates.txt").write_text(" ".join("%.17g" % v for v in r))
+    env = dict(os.environ, TASEP_INTEGRATOR="rk4", TASEP_PRECISION=prec)
+    p = subprocess.run([BIN, "-n", str(n), "-i", "rates.txt", "-t", "0", "-T", str(T), "-s", str(steps),
+                        "-o", "out"], cwd=tmp_path, capture_output=True, text=True, timeout=300, env=env)
+    assert p.returncode == 0, p.stderr
+    h = oracle.step_h(T, 0.0, steps)
+    ox, _ = oracle.run_rk4(n, r, oracle.uniform_x0(n), 0.0, h, steps)
+    y = oracle.site_moments(n, ox)
+    vec = [float(v) for v in open(tmp_path / "out_exp_values_fin.vec").read().split(",")]
+    assert len(vec) == n
+    assert max(abs(vec[i] - y[n - 1 - i]) for i in range(n)) <= 1.5e-6
+    vals = np.array([float(ln.split("\t\t")[1]) for ln in open(tmp_path / "out")], np.float64)
+    assert np.max(np.abs(vals - ox)) <= (1e-12 if prec == "f64" else 1e-9)
+
+
+@needs_bin
+@pytest.mark.gpu
 @pytest.mark.parametrize("n,fast", [(14, False), (20, False), (20, True)])
 def test_host_large_lattice_vs_oracle(tmp_path, n, fast):
     """Larger lattices than the fixtures (other kernels: one lane per state,
