@@ -177,7 +177,8 @@ int tasep_step_profile(tasep_ctx *ctx, double t0, double h, int reps,
  *                              step, 3 work vectors instead of 5)
  *   TASEP_OPT_TILE_BLOCKS  [0] workgroups of the tiled passes (multiple of
  *                              8); 0 = 16 per workgroup slot resident at
- *                              once (at most 8192)
+ *                              once, or 8192 for RK4 passes over at least
+ *                              65536 tiles
  *   TASEP_OPT_LDS          [1] matrix-free runs that keep two copies of the
  *                              state vector in one workgroup's LDS and run
  *                              all their steps there, in one launch per

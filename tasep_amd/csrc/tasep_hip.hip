@@ -498,18 +498,25 @@ static int resident_slots(const void *kernel) {
 // Grid and tile order of a tiled pass.  Grid: 16 workgroups per resident
 // slot, which balances the tail (measured at n=28: 0.92 vs 1.03 ms for one
 // resident wave), at most kMaxBlocks partials for the end-of-step
-// reduction, rounded down to a multiple of 8 XCDs.  Order (tile_walk): the
-// requested one where the grid allows it, else the next simpler one.
+// reduction, rounded down to a multiple of 8 XCDs.  The RK4 passes (`wide`)
+// take kMaxBlocks workgroups once that still gives each 8 tiles or more
+// (per step, 8192 vs 4096 workgroups: FP64 n=28 6.84 vs 6.92 ms, n=30
+// 32.7 vs 33.7 ms, FP32 n=30 15.5 vs 15.6 ms, n=32 73.7 vs 75.0 ms; with 4
+// tiles each, FP32 n=28 and FP64 n=27, 0.6-2.4% slower; the reference loop
+// at n=28 0.9% slower).  Order (tile_walk): the requested one where the
+// grid allows it, else the next simpler one.
 struct TileLaunch {
     uint64_t grid;
     int order;
     uint32_t res;
 };
-static TileLaunch tile_launch(const tasep_ctx *c, const void *kernel, int &slots_cache, uint64_t ntiles) {
+static TileLaunch tile_launch(const tasep_ctx *c, const void *kernel, int &slots_cache, uint64_t ntiles,
+                              bool wide = false) {
     if (!slots_cache) slots_cache = resident_slots(kernel);
     const int slots = slots_cache;
     uint64_t cap = slots > 0 ? ((uint64_t)slots * 16) & ~(uint64_t)7 : 1024;
     if (cap < 8) cap = 8;
+    if (wide && ntiles >= (uint64_t)kMaxBlocks * 8) cap = kMaxBlocks;
     if (cap > (uint64_t)kMaxBlocks) cap = kMaxBlocks;
     if (c->tile_blocks > 0) cap = (uint64_t)c->tile_blocks;
     TileLaunch L;
@@ -529,7 +536,7 @@ static TileLaunch tile_launch(const tasep_ctx *c, const void *kernel, int &slots
 template <class V, int EPI, bool SELF, bool NT>
 static uint32_t launch_tile_nt(tasep_ctx *c, TileArgs<V> a) {
     static int slots = 0;
-    const TileLaunch L = tile_launch(c, (const void *)k_tile_pass<V, EPI, SELF, NT>, slots, a.ntiles);
+    const TileLaunch L = tile_launch(c, (const void *)k_tile_pass<V, EPI, SELF, NT>, slots, a.ntiles, true);
     a.xcd_order = L.order;
     a.xcd_res = L.res;
     hipLaunchKernelGGL((k_tile_pass<V, EPI, SELF, NT>), dim3(L.grid), dim3(kTileThreads), 0, c->stream, a);
