@@ -176,9 +176,10 @@ int tasep_step_profile(tasep_ctx *ctx, double t0, double h, int reps,
  *                              rates (default: the Horner form of the same
  *                              step, 3 work vectors instead of 5)
  *   TASEP_OPT_TILE_BLOCKS  [0] workgroups of the tiled passes (multiple of
- *                              8); 0 = 16 per workgroup slot resident at
- *                              once, or 8192 for RK4 passes over at least
- *                              65536 tiles
+ *                              8, at most 16384); 0 = 16 per workgroup slot
+ *                              resident at once, or for RK4 passes over at
+ *                              least 65536 tiles the largest of 8192 and
+ *                              16384 that leaves 8 tiles per workgroup
  *   TASEP_OPT_LDS          [1] matrix-free runs that keep two copies of the
  *                              state vector in one workgroup's LDS and run
  *                              all their steps there, in one launch per

@@ -13,7 +13,7 @@
 
 namespace tasep_host {
 
-constexpr int kMaxBlocks = 8192;  // grid cap of the streaming kernels
+constexpr int kMaxBlocks = 16384;  // grid cap of the streaming kernels
 constexpr int kWindow = 128;      // steps per reduction window with the exit test armed
 constexpr int kMomMaxBlocks = 2048;
 

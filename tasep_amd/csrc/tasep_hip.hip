@@ -499,11 +499,12 @@ static int resident_slots(const void *kernel) {
 // slot, which balances the tail (measured at n=28: 0.92 vs 1.03 ms for one
 // resident wave), at most kMaxBlocks partials for the end-of-step
 // reduction, rounded down to a multiple of 8 XCDs.  The RK4 passes (`wide`)
-// take kMaxBlocks workgroups once that still gives each 8 tiles or more
-// (per step, 8192 vs 4096 workgroups: FP64 n=28 6.84 vs 6.92 ms, n=30
-// 32.7 vs 33.7 ms, FP32 n=30 15.5 vs 15.6 ms, n=32 73.7 vs 75.0 ms; with 4
-// tiles each, FP32 n=28 and FP64 n=27, 0.6-2.4% slower; the reference loop
-// at n=28 0.9% slower).  Order (tile_walk): the requested one where the
+// take the largest power of two from 8192 up to kMaxBlocks workgroups that
+// still gives each 8 tiles or more (per step, 8192 vs 4096 workgroups: FP64
+// n=28 6.84 vs 6.92 ms, n=30 32.7 vs 33.7 ms, FP32 n=30 15.5 vs 15.6 ms;
+// 16384 vs 8192 at FP32 n=32: 69.4-71.6 vs 72.4-74.2 ms; with 4 tiles each,
+// FP32 n=28 and FP64 n=27, 0.6-2.4% slower; the reference loop at n=28
+// 0.9% slower).  Order (tile_walk): the requested one where the
 // grid allows it, else the next simpler one.
 struct TileLaunch {
     uint64_t grid;
@@ -516,7 +517,11 @@ static TileLaunch tile_launch(const tasep_ctx *c, const void *kernel, int &slots
     const int slots = slots_cache;
     uint64_t cap = slots > 0 ? ((uint64_t)slots * 16) & ~(uint64_t)7 : 1024;
     if (cap < 8) cap = 8;
-    if (wide && ntiles >= (uint64_t)kMaxBlocks * 8) cap = kMaxBlocks;
+    if (wide) {
+        uint64_t w = 8192;
+        while (w * 2 <= (uint64_t)kMaxBlocks && ntiles >= w * 2 * 8) w *= 2;
+        if (ntiles >= w * 8) cap = w;
+    }
     if (cap > (uint64_t)kMaxBlocks) cap = kMaxBlocks;
     if (c->tile_blocks > 0) cap = (uint64_t)c->tile_blocks;
     TileLaunch L;
@@ -935,7 +940,7 @@ int tasep_set_option(tasep_ctx *c, int key, int64_t value) {
         break;
     case TASEP_OPT_TILE_BLOCKS:
         if (value != 0 && (value < 8 || value > kMaxBlocks || value % 8))
-            return fail(TASEP_ERR_ARG, "tile blocks: 0 or a multiple of 8 in [8, 8192]");
+            return fail(TASEP_ERR_ARG, "tile blocks: 0 or a multiple of 8 in [8, 16384]");
         c->tile_blocks = (int)value;
         break;
     default: return fail(TASEP_ERR_ARG, "unknown option");
