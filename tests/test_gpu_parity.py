@@ -376,6 +376,23 @@ def test_full_size_uniform_is_stationary(n, prec):
     assert abs(mass - 1.0) <= (1e-5 if prec == "f32" else 1e-12)
 
 
+def test_ref_loop_largest_fp64_lattice_n33_uniform_is_stationary():
+    """The reference loop on FP64 tiles at n=33 (64 GiB per vector, 21
+    tile-index bits): alpha = beta = 1/2, gamma = 1 leaves the uniform state
+    exactly unchanged -- every row sums exact multiples of x to 0 -- so the
+    checksum is exactly 0 and the densities are 1/2."""
+    n = 33
+    r = np.array([0.5] + [1.0] * (n - 1) + [0.5])
+    with T.Engine(n, "f64", "ref", "mf") as e:
+        e.set_rates(r)
+        e.set_x0_uniform()
+        info = e.run(0.0, 0.01, 3, patience=-1)
+        y, cov, _, mass = e.moments()
+    assert info.steps_done == 3 and info.last_checksum == 0.0
+    assert np.max(np.abs(y - 0.5)) <= 1e-12
+    assert abs(mass - 1.0) <= 1e-12
+
+
 def test_full_size_n28_ref_tiles_bitwise_vs_per_state_kernel():
     """The bit-exact reference loop at the n=28 benchmark size: the tiled
     kernel (per-XCD sweep, pre-issued streams, hoisted diagonal, division by
