@@ -318,3 +318,122 @@ def test_high_order_files_from_windows_match_reference(name):
     W = oracle.windows(c["n"], c["x"])
     for fname, text in H.files(W, c["n"]).items():
         assert text == G.read_text(c, fname), fname
+
+
+# ------------------------------------------- time-variant rates and RK4 (a6)
+def _tv_cases():
+    """Rate functions of all three types, with nonzero offsets, amplitudes
+    of both signs and constants small enough that the clamp at 0 acts."""
+    rng = np.random.default_rng(44)
+    m = 24
+    return dict(type=np.array([0, 1, 2] * (m // 3), np.int32),
+                amp=rng.uniform(-1.5, 1.5, m), period=rng.uniform(0.3, 3.0, m),
+                offset=rng.uniform(-2.0, 2.0, m), c=rng.uniform(-0.5, 1.2, m))
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="oracle/_ref not built here")
+def test_tv_rates_bit_exact_vs_reference_timeVariantProbs():
+    """oracle.tv_rates against the reference's own timeVariantProbs
+    (src/lib/matvec.c:44-68, compiled from source): sin, cos and linear
+    types, the clamp at 0, nonzero offsets, and an unknown type that leaves
+    the rate as it was."""
+    R = oracle.ref_lib()
+    tv = _tv_cases()
+    m = len(tv["type"])
+    clamped = 0
+    for t in (0.0, 0.37, 1.0, 2.5, 7.25, -1.5, 123.456):
+        want = np.zeros(m)
+        R.timeVariantProbs(want, m, np.ascontiguousarray(tv["period"]), np.ascontiguousarray(tv["offset"]),
+                           np.ascontiguousarray(tv["amp"]), t, tv["type"], np.ascontiguousarray(tv["c"]))
+        got = oracle.tv_rates(m - 1, t, tv)
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), t
+        clamped += int(np.sum(want == 0.0))
+        assert np.all(want >= 0)
+    assert clamped > 10                               # the clamp was exercised
+    # closed forms of the three types (clamp included)
+    t = 0.9
+    got = oracle.tv_rates(m - 1, t, tv)
+    arg = t * tv["period"] + tv["offset"]
+    raw = np.where(tv["type"] == 0, tv["amp"] * np.sin(arg) + tv["c"],
+                   np.where(tv["type"] == 1, tv["amp"] * np.cos(arg) + tv["c"], tv["amp"] * t + tv["c"]))
+    assert np.allclose(got, np.maximum(raw, 0.0), rtol=0, atol=1e-15)
+    # an unknown type leaves the caller's rate untouched, as the reference's switch does
+    tv3 = dict(tv, type=np.full(m, 3, np.int32))
+    keep = np.full(m, 0.625)
+    R.timeVariantProbs(keep, m, np.ascontiguousarray(tv["period"]), np.ascontiguousarray(tv["offset"]),
+                       np.ascontiguousarray(tv["amp"]), 1.0, tv3["type"], np.ascontiguousarray(tv["c"]))
+    assert np.all(keep == 0.625)
+
+
+def _tv_problem(n, seed):
+    rng = np.random.default_rng(seed)
+    return dict(type=(np.arange(n + 1) % 3).astype(np.int32), amp=rng.uniform(0.4, 0.9, n + 1),
+                period=rng.uniform(1.5, 3.0, n + 1), offset=rng.uniform(-1.0, 1.0, n + 1),
+                c=rng.uniform(1.0, 1.5, n + 1))
+
+
+def _exact_tv_solution(n, tv, x0, T):
+    """x(T) of dx/dt = L(r(t)) x from scipy's 8th-order integrator at a tight
+    tolerance; L from the oracle's dense generator (pinned above against the
+    reference's gencrs), r(t) from oracle.tv_rates (pinned above against the
+    reference's timeVariantProbs)."""
+    from scipy.integrate import solve_ivp
+    sol = solve_ivp(lambda t, x: oracle.dense(n, oracle.tv_rates(n, t, tv)) @ x, (0.0, T), x0,
+                    method="DOP853", rtol=1e-13, atol=1e-16)
+    assert sol.success
+    return sol.y[:, -1]
+
+
+def _rk4_dense(n, tv, x0, T, steps, stage_t):
+    """Classical RK4 on the dense generator with stage times t + stage_t[i]*h
+    (correct: 0, 1/2, 1/2, 1).  Only used to show that the convergence check
+    below tells a wrong stage time apart."""
+    h = T / steps
+    x = x0.copy()
+    t = 0.0
+    for _ in range(steps):
+        Ls = [oracle.dense(n, oracle.tv_rates(n, t + c * h, tv)) for c in stage_t]
+        k1 = h * (Ls[0] @ x)
+        k2 = h * (Ls[1] @ (x + 0.5 * k1))
+        k3 = h * (Ls[2] @ (x + 0.5 * k2))
+        k4 = h * (Ls[3] @ (x + k3))
+        x = x + (k1 + 2 * k2 + 2 * k3 + k4) / 6
+        t += h
+    return x
+
+
+def _order_ratios(errs):
+    return [errs[i] / errs[i + 1] for i in range(len(errs) - 1)]
+
+
+def test_rk4_time_variant_oracle_is_fourth_order():
+    """The oracle's classical RK4 with time-variant rates (stages at t,
+    t+h/2, t+h/2, t+h) converges at fourth order to the exact solution: the
+    error falls ~16x per halving of h.  The same check applied to RK4 with
+    stages 2 and 3 at t or at t + h (or the first or last stage at t + h/2)
+    falls only ~2x, so a stage-time mistake in the oracle fails it."""
+    n = 4
+    tv = _tv_problem(n, 7)
+    x0 = np.zeros(1 << n)
+    x0[0] = 1.0
+    T = 2.0
+    exact = _exact_tv_solution(n, tv, x0, T)
+    steps = (20, 40, 80)
+    errs = []
+    for s in steps:
+        x, info = oracle.run_rk4(n, None, x0, 0.0, T / s, s, tv=tv)
+        assert info.steps_done == s and abs(info.t - T) < 1e-12
+        errs.append(float(np.max(np.abs(x - exact))))
+    ratios = _order_ratios(errs)
+    assert all(q > 13 for q in ratios), (errs, ratios)
+    assert errs[-1] < 5e-8
+    # the dense restatement with the right stage times agrees with the oracle
+    x_d = _rk4_dense(n, tv, x0, T, 40, (0.0, 0.5, 0.5, 1.0))
+    x_o, _ = oracle.run_rk4(n, None, x0, 0.0, T / 40, 40, tv=tv)
+    assert np.max(np.abs(x_d - x_o)) < 1e-14
+    # planted mistakes (in this CPU restatement only): stages 2, 3 at t or at t + h
+    # and a last or first stage at t + h/2: each of them is first order
+    for wrong in ((0.0, 0.0, 0.0, 1.0), (0.0, 1.0, 1.0, 1.0), (0.0, 0.5, 0.5, 0.5),
+                  (0.5, 0.5, 0.5, 1.0)):
+        e = [float(np.max(np.abs(_rk4_dense(n, tv, x0, T, s, wrong) - exact))) for s in steps]
+        assert max(_order_ratios(e)) < 4, (wrong, e)
