@@ -3,8 +3,9 @@
 
 Workload (n=28 precision configuration): 2^28 occupation states, alpha and
 beta ~ U[0.2, 1.0], 27 bulk rates ~ LogNormal(0, 0.3) (seed 28), uniform
-initial state, h = 0.01, classical RK4 with an FP32 state vector (FP64
-arithmetic, sums and renormalisation), matrix-free generator.  A "step" is
+initial state, h = 0.01, classical RK4 with an FP32 state vector and FP32
+generator arithmetic (FP64 checksum, mass and moments; the moments are
+divided by the FP64 mass), matrix-free generator.  A "step" is
 one RK4 step over all 2^28 states; the metric is state-updates/s =
 steps * 2^n / s with the state resident in HBM.
 

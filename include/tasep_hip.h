@@ -38,8 +38,9 @@ extern "C" {
 
 /* precision of the state vector */
 #define TASEP_F64   0  /* FP64 vector, FP64 arithmetic                        */
-#define TASEP_F32   1  /* FP32 vector; generator sums, stage combination,
-                          checksum, mass and moments in FP64                  */
+#define TASEP_F32   1  /* FP32 vector and FP32 generator rows, stage
+                          combination and update in every kernel; checksum,
+                          mass, moments and time-variant rates in FP64        */
 /* integrator */
 #define TASEP_REF   0  /* the reference's loop: all four stages read the
                           unmodified x (src/lib/solv.c:12-13), update
@@ -85,8 +86,19 @@ int tasep_set_time_variant(tasep_ctx *ctx, const int32_t *type,
 int tasep_set_x0_uniform(tasep_ctx *ctx);
 int tasep_set_x0_state(tasep_ctx *ctx, uint64_t s);
 int tasep_set_x(tasep_ctx *ctx, const double *x);
+/* Product measure x_s = prod_k (b_k(s) ? p[k] : 1 - p[k]) over the n bits
+ * (p[k] = occupation probability of site n-k, in [0, 1]), built on the
+ * device in FP64 with k ascending.  p[k] = 1/2 for every k is the
+ * reference's uniform start (src/masterEquation.c:402-409); the reference
+ * has no other (its x0 is edited in the source, :411). */
+int tasep_set_x0_product(tasep_ctx *ctx, const double *p);
 /* Copy the current state to a host FP64 vector of 2^n entries. */
 int tasep_get_x(tasep_ctx *ctx, double *x);
+/* Copy entries [offset, offset + count) of the current state to host FP64
+ * (reads a large state in pieces, as the probabilities file of
+ * src/masterEquation.c:1021-1040 walks it in order). */
+int tasep_get_x_range(tasep_ctx *ctx, uint64_t offset, uint64_t count,
+                      double *x);
 
 /* Build the CSR generator on the device (replaces gencrs,
  * src/lib/matgen.c:7-74; arrays bit-identical to it).  Requires rates. */

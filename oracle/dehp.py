@@ -18,6 +18,21 @@ of ``(alpha/gamma, beta/gamma, 1)``.  Inputs are taken as the exact binary
 values of the given doubles (the reference rounds 1/alpha to a double first,
 :118-119; this module keeps 1/alpha exact).
 
+Two-point function (``two_point_function``, :188-326), sites 1-based from the
+entry, i < j < N:
+
+* ``<tau_i tau_j>_N = sum_{p=0}^{N-j-1} C_p (Z_{N-1-p}/Z_N) <tau_i>_{N-1-p}
+  + (Z_{j-1}/Z_N) <tau_i>_{j-1} sum_{p=2}^{N-j+1} (p-1)(2(N-j)-p)! /
+  ((N-j)! (N-j+1-p)!) beta^-p``, C_p = (2p)!/(p!(p+1)!) (``two_point``);
+  j = N, which the reference excludes (``assert(j<N)``, :191), from the
+  boundary vector: ``<tau_i tau_N>_N = Z_{N-1} <tau_i>_{N-1} / (beta Z_N)``.
+* The reference's own sums take every Z through ``calculate_Z_i(m)``
+  (:329-340), which returns ``Z[m-1]`` of an array whose entry m is Z_m, so
+  each Z in its result is one index lower than in the formula above.
+  ``two_point_reference`` restates that literally; it equals the exact pair
+  moment only where the Z are geometric (alpha + beta = 1), and it is what
+  ``steadystates_gmp -i -j`` prints.
+
 ``steady_state_bruteforce`` solves ``L p = 0`` exactly over Fractions for
 small n from the generator definition alone; tests use it to pin this module.
 """
@@ -67,6 +82,51 @@ def densities(n: int, alpha, beta, gamma=1):
         rho.append(acc / Z[n])
     rho.append(Z[n - 1] / (be * Z[n]))
     return rho
+
+
+def _catalan(p: int) -> Fraction:
+    return Fraction(factorial(2 * p), factorial(p) * factorial(p + 1))
+
+
+def _boundary_sum(N: int, j: int, b: Fraction) -> Fraction:
+    """sum_{p=2}^{N-j+1} (p-1)(2(N-j)-p)! / ((N-j)!(N-j+1-p)!) b^p, b = 1/beta
+    (the second sum of steadystates_gmp.c:283-318)."""
+    m = N - j
+    acc = Fraction(0)
+    for p in range(2, m + 2):
+        acc += Fraction((p - 1) * factorial(2 * m - p), factorial(m) * factorial(m + 1 - p)) * b ** p
+    return acc
+
+
+def _two_point(i: int, j: int, alpha, beta, N: int, gamma, zshift: int) -> Fraction:
+    g = _F(gamma)
+    al, be = _F(alpha) / g, _F(beta) / g
+    if not (1 <= i < j <= N):
+        raise ValueError("need 1 <= i < j <= N")
+    Zs = Z_list(N, al, be)
+    Z = lambda m: Zs[m - zshift]  # noqa: E731
+    if j == N and zshift == 0:
+        return Zs[N - 1] * densities(N - 1, al, be)[i - 1] / (be * Zs[N])
+    if j == N:
+        raise ValueError("the reference's two-point function needs j < N")
+    first = Fraction(0)
+    for p in range(0, N - j):                        # :230-279
+        first += _catalan(p) * Z(N - 1 - p) / Z(N) * densities(N - p - 1, al, be)[i - 1]
+    second = _boundary_sum(N, j, 1 / be) * Z(j - 1) * densities(j - 1, al, be)[i - 1] / Z(N)
+    return first + second
+
+
+def two_point(i: int, j: int, alpha, beta, N: int, gamma=1) -> Fraction:
+    """Exact stationary <tau_i tau_j> (sites 1-based from the entry,
+    1 <= i < j <= N), as a Fraction."""
+    return _two_point(i, j, alpha, beta, N, gamma, 0)
+
+
+def two_point_reference(i: int, j: int, alpha, beta, N: int) -> Fraction:
+    """The value steadystates_gmp's two_point_function computes (exact
+    arithmetic in place of 64-bit mpf), with its Z index one lower than the
+    formula's (calculate_Z_i, :329-340); 1 <= i < j < N."""
+    return _two_point(i, j, alpha, beta, N, 1, 1)
 
 
 def current(n: int, alpha, beta, gamma=1):

@@ -36,6 +36,7 @@ EXPORTS = [
     "tasep_batch_get_x", "tasep_batch_run", "tasep_batch_moments",
     "tasep_batch_set_option", "tasep_batch_get_stream", "tasep_window_marginals",
     "tasep_batch_window_marginals", "tasep_batch_get_option", "tasep_window_marginals_w",
+    "tasep_set_x0_product", "tasep_get_x_range",
 ]
 OPT_TILED, OPT_XCD_ORDER, OPT_RK4_STAGED, OPT_TILE_BLOCKS, OPT_LDS, OPT_LDS_THREADS = 1, 2, 3, 4, 5, 6
 OPT_GRAPH, OPT_STORE = 7, 8
@@ -98,6 +99,8 @@ def load_library():
     L.tasep_set_x0_state.argtypes = [vp, C.c_uint64]
     L.tasep_set_x.argtypes = [vp, _f64p]
     L.tasep_get_x.argtypes = [vp, _f64p]
+    L.tasep_set_x0_product.argtypes = [vp, _f64p]
+    L.tasep_get_x_range.argtypes = [vp, C.c_uint64, C.c_uint64, _f64p]
     L.tasep_build_csr.argtypes = [vp]
     L.tasep_get_csr.argtypes = [vp, _u64p, _u64p, _f64p]
     L.tasep_gencrs.argtypes = [_u64p, _f64p, _u64p, _f64p, C.c_uint64, C.c_int]
@@ -208,6 +211,13 @@ class Engine:
     def set_x0_state(self, s: int):
         _check(load_library().tasep_set_x0_state(self._h, s))
 
+    def set_x0_product(self, p):
+        """x_s = prod_k (b_k(s) ? p[k] : 1 - p[k]); p[k] for bit k (site n-k)."""
+        p = np.ascontiguousarray(p, np.float64)
+        if p.shape != (self.n,):
+            raise ValueError(f"need {self.n} site densities")
+        _check(load_library().tasep_set_x0_product(self._h, p))
+
     def set_x(self, x):
         x = np.ascontiguousarray(x, np.float64)
         if x.shape != (self.dim,):
@@ -217,6 +227,11 @@ class Engine:
     def get_x(self):
         x = np.empty(self.dim, np.float64)
         _check(load_library().tasep_get_x(self._h, x))
+        return x
+
+    def get_x_range(self, offset: int, count: int):
+        x = np.empty(count, np.float64)
+        _check(load_library().tasep_get_x_range(self._h, offset, count, x))
         return x
 
     def set_option(self, key: int, value: int):

@@ -24,14 +24,12 @@ struct RunState {
     double thr_h;        // early-exit threshold thr*h
     double checksum;     // sum |dx| of the last step
     double mass;         // sum x after the last step
-    double scale;        // FP32 path: 1/mass of the stored vector (else 1)
     uint64_t steps_done;
     uint64_t steps_at_stop;
     int32_t early_stop;
     int32_t patience;    // < 0: never leave early
     int32_t stopped;
-    int32_t renorm;      // 1: scale <- 1/mass after each step
-    uint32_t pad0, pad1;
+    uint32_t pad0;
     double t_rates;      // start time of the next step whose rates k_tv_rates evaluates
 };
 
@@ -88,7 +86,6 @@ __device__ __forceinline__ void advance_state(RunState *st, double chk, double m
         }
         if (st->early_stop > st->patience) st->stopped = 1;
     }
-    if (st->renorm && mass > 0.0) st->scale = 1.0 / mass;
 }
 
 // Fixed-order FP64 sums of nslots block partials (every thread of the

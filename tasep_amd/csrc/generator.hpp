@@ -92,30 +92,30 @@ __device__ __forceinline__ double ref_delta(double h, double T) {
 }
 
 // ---------------------------------------------------------------------------
-// Performance rows (TASEP_RK4): same terms, FP64 arithmetic with explicit
-// FMA, stage input supplied by a gather functor G(j) -> double.
+// Performance rows (TASEP_RK4): same terms with explicit FMA, computed in the
+// arithmetic type A (the state's storage type), stage input supplied by a
+// gather functor G(j) -> A.
 // ---------------------------------------------------------------------------
-template <class G>
-__device__ __forceinline__ double rk_row_mf(const Lattice g, uint64_t s, const G &u,
-                                            const double *r) {
+template <class A, class G>
+__device__ __forceinline__ A rk_row_mf(const Lattice g, uint64_t s, const G &u, const double *r) {
     const int n = g.n;
     const bool occ_top = (s & g.top) != 0, occ_exit = (s & 1) != 0;
-    double d = occ_exit ? r[n] : 0.0;
-    if (!occ_top) d += r[0];
+    A d = occ_exit ? (A)r[n] : (A)0;
+    if (!occ_top) d += (A)r[0];
     uint64_t mo = ~s & (s >> 1) & g.bulk;
     while (mo) {
         int k = __builtin_ctzll(mo);
         mo &= mo - 1;
-        d += r[n - 1 - k];
+        d += (A)r[n - 1 - k];
     }
-    double acc = -d * u(s);
-    if (occ_top) acc = __builtin_fma(r[0], u(s - g.top), acc);
-    if (!occ_exit) acc = __builtin_fma(r[n], u(s + 1), acc);
+    A acc = -d * u(s);
+    if (occ_top) acc = fma((A)r[0], u(s - g.top), acc);
+    if (!occ_exit) acc = fma((A)r[n], u(s + 1), acc);
     uint64_t mi = s & ~(s >> 1) & g.bulk;
     while (mi) {
         int k = __builtin_ctzll(mi);
         mi &= mi - 1;
-        acc = __builtin_fma(r[n - 1 - k], u(s + ((uint64_t)1 << k)), acc);
+        acc = fma((A)r[n - 1 - k], u(s + ((uint64_t)1 << k)), acc);
     }
     return acc;
 }

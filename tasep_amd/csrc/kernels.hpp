@@ -184,9 +184,10 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish_step(RunState *st, co
 //    P1: U = X + h/4 L X   P2: V = X + h/3 L U   P3: U = X + h/2 L V
 //    P4: X = X + h L U  (in place: X is read only at s)
 //
-// The gathered stage input is G(j) = gs*GX[j] + ck*GK[j]; the self term is
-// sc*X[s], where sc = RunState::scale renormalises FP32 vectors by the FP64
-// mass of the previous step.  All arithmetic is FP64; V is the storage type.
+// The gathered stage input is G(j) = GX[j] + ck*GK[j], the self term X[s].
+// Rows, stage combination and update are computed in the storage type V
+// (FP32 contexts: FP32, as in the tiled, LDS-resident and batch kernels);
+// the checksum and mass are FP64 sums.
 // ===========================================================================
 // EPI_STORE: out = self + c * (L G).  EPI_HORNER: x += c * (L G) with the
 // step's checksum and mass.  EPI_RK4: the staged step's combination from
@@ -199,8 +200,7 @@ struct PassArgs {
     const V *gx;      // gathered vector
     const V *gk;      // optional second gathered vector
     double ck;        // weight of gk
-    int gx_is_x;      // 1: gx is the state (scaled by sc)
-    int use_self;     // STORE: include a_self*sc*X[s]
+    int use_self;     // STORE: include X[s]
     const V *x;       // state (self term)
     V *out;
     const V *k1, *k2, *k3;
@@ -220,18 +220,15 @@ struct PassArgs {
 template <class V>
 struct Gather1 {
     const V *__restrict__ p;
-    double w;
-    __device__ __forceinline__ double operator()(uint64_t j) const { return w * (double)p[j]; }
+    __device__ __forceinline__ V operator()(uint64_t j) const { return p[j]; }
 };
 
 template <class V>
 struct Gather2 {
     const V *__restrict__ p;
     const V *__restrict__ q;
-    double w, ck;
-    __device__ __forceinline__ double operator()(uint64_t j) const {
-        return __builtin_fma(ck, (double)q[j], w * (double)p[j]);
-    }
+    V ck;
+    __device__ __forceinline__ V operator()(uint64_t j) const { return fma(ck, q[j], p[j]); }
 };
 
 // SELF: the self term is read from the gathered vector (Horner P1, where the
@@ -242,35 +239,34 @@ __global__ __launch_bounds__(kBlock) void k_rk_pass(const PassArgs<V> a) {
     __shared__ double r[kRateStride];
     __shared__ double red[kWaves];
     stage_rates(r, a.rates, a.rset, a.g.n);
-    const double sc = a.st->scale;
-    const double w = a.gx_is_x ? sc : 1.0;
+    const V cmul = (V)a.c;
     Kahan chk, mass;
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     for (uint64_t s = (uint64_t)blockIdx.x * kBlock + threadIdx.x; s < a.dim; s += stride) {
-        double acc;
+        V acc;
         if constexpr (TWO) {
-            Gather2<V> G{a.gx, a.gk, w, a.ck};
-            acc = CSR ? rk_row_csr(s, a.rowPtr, a.colInd, a.val, G) : rk_row_mf(a.g, s, G, r);
+            Gather2<V> G{a.gx, a.gk, (V)a.ck};
+            acc = CSR ? rk_row_csr(s, a.rowPtr, a.colInd, a.val, G) : rk_row_mf<V>(a.g, s, G, r);
         } else {
-            Gather1<V> G{a.gx, w};
-            acc = CSR ? rk_row_csr(s, a.rowPtr, a.colInd, a.val, G) : rk_row_mf(a.g, s, G, r);
+            Gather1<V> G{a.gx};
+            acc = CSR ? rk_row_csr(s, a.rowPtr, a.colInd, a.val, G) : rk_row_mf<V>(a.g, s, G, r);
         }
         if constexpr (EPI == EPI_STORE) {
-            double o = a.c * acc;
-            if constexpr (SELF) o += sc * (double)a.gx[s];
-            else if (a.use_self) o += sc * (double)a.x[s];
-            a.out[s] = (V)o;
-        } else {
-            double d;
-            if constexpr (EPI == EPI_HORNER) {
-                d = a.c * acc;
-            } else {
-                const double k4 = a.c * acc;
-                d = (((double)a.k1[s] + 2.0 * (double)a.k2[s]) + 2.0 * (double)a.k3[s] + k4) / 6.0;
-            }
-            const V o = (V)(sc * (double)a.x[s] + d);
+            V o = cmul * acc;
+            if constexpr (SELF) o += a.gx[s];
+            else if (a.use_self) o += a.x[s];
             a.out[s] = o;
-            chk.add(fabs(d));
+        } else {
+            V d;
+            if constexpr (EPI == EPI_HORNER) {
+                d = cmul * acc;
+            } else {
+                const V k4 = cmul * acc;
+                d = ((a.k1[s] + (V)2 * a.k2[s]) + (V)2 * a.k3[s] + k4) / (V)6;
+            }
+            const V o = a.x[s] + d;
+            a.out[s] = o;
+            chk.add(fabs((double)d));
             mass.add((double)o);
         }
     }
@@ -586,6 +582,22 @@ __global__ void k_convert(const A *__restrict__ src, B *__restrict__ dst, uint64
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < dim; s += stride)
         dst[s] = (B)src[s];
+}
+
+// Product measure x_s = prod_k (b_k(s) ? p_k : 1 - p_k), k ascending, in
+// FP64 (rounded once to V).
+struct ProductArgs {
+    double p[kMaxN];
+};
+
+template <class V>
+__global__ void k_product_state(V *__restrict__ x, uint64_t dim, int n, const ProductArgs a) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < dim; s += stride) {
+        double v = 1.0;
+        for (int k = 0; k < n; k++) v *= ((s >> k) & 1) ? a.p[k] : 1.0 - a.p[k];
+        x[s] = (V)v;
+    }
 }
 
 }  // namespace tasep

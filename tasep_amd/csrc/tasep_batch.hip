@@ -540,7 +540,6 @@ int tasep_batch_run(tasep_batch *b, double t0, double h, uint64_t steps, double 
         s.h = h;
         s.thr_h = thr * h;
         s.patience = b->tv[i] ? -1 : patience;
-        s.scale = 1.0;
     }
     HIP_OK(hipMemcpyAsync(b->st, b->st_host, (size_t)b->count * sizeof(RunState), hipMemcpyHostToDevice,
                           b->stream));

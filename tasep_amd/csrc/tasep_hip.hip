@@ -169,7 +169,6 @@ int tasep_ctx_create(tasep_ctx **out, int n, int precision, int integrator, int 
              "hipHostMalloc(instance)"));
     CK(ensure_buf(c, 0));
     memset(c->st_host, 0, sizeof(RunState));
-    c->st_host->scale = 1.0;
     CK(hipck(hipMemcpyAsync(c->st, c->st_host, sizeof(RunState), hipMemcpyHostToDevice, c->stream),
              "hipMemcpy(state)"));
     if (c->prec == TASEP_F32) launch_fill<float>(c, c->buf[0], 1.0 / (double)c->dim);
@@ -218,21 +217,11 @@ int tasep_set_time_variant(tasep_ctx *c, const int32_t *type, const double *amp,
     return TASEP_OK;
 }
 
-static int reset_scale(tasep_ctx *c) {
-    // a freshly set state is its own normalisation reference
-    c->st_host->scale = 1.0;
-    HIP_OK(hipMemcpyAsync(&c->st->scale, &c->st_host->scale, sizeof(double), hipMemcpyHostToDevice,
-                          c->stream));
-    return TASEP_OK;
-}
-
 int tasep_set_x0_uniform(tasep_ctx *c) {
     if (!c) return fail(TASEP_ERR_ARG, "NULL context");
     c->cur = 0;
     if (c->prec == TASEP_F32) launch_fill<float>(c, xcur(c), 1.0 / (double)c->dim);
     else launch_fill<double>(c, xcur(c), 1.0 / (double)c->dim);
-    int rc = reset_scale(c);
-    if (rc) return rc;
     HIP_OK(hipStreamSynchronize(c->stream));
     return TASEP_OK;
 }
@@ -249,8 +238,6 @@ int tasep_set_x0_state(tasep_ctx *c, uint64_t s) {
         double one = 1.0;
         HIP_OK(hipMemcpyAsync((double *)xcur(c) + s, &one, 8, hipMemcpyHostToDevice, c->stream));
     }
-    int rc = reset_scale(c);
-    if (rc) return rc;
     HIP_OK(hipStreamSynchronize(c->stream));
     return TASEP_OK;
 }
@@ -268,8 +255,6 @@ int tasep_set_x(tasep_ctx *c, const double *x) {
                            tmp, (float *)xcur(c), c->dim);
         HIP_OK(hipFreeAsync(tmp, c->stream));
     }
-    int rc = reset_scale(c);
-    if (rc) return rc;
     HIP_OK(hipStreamSynchronize(c->stream));
     return TASEP_OK;
 }
@@ -284,6 +269,44 @@ int tasep_get_x(tasep_ctx *c, double *x) {
         hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(c->dim)), dim3(kBlock), 0, c->stream,
                            (const float *)xcur(c), tmp, c->dim);
         HIP_OK(hipMemcpyAsync(x, tmp, c->dim * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipFreeAsync(tmp, c->stream));
+    }
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return TASEP_OK;
+}
+
+int tasep_set_x0_product(tasep_ctx *c, const double *p) {
+    if (!c || !p) return fail(TASEP_ERR_ARG, "NULL argument");
+    ProductArgs a;
+    memset(&a, 0, sizeof a);
+    for (int k = 0; k < c->n; k++) {
+        if (!(p[k] >= 0.0 && p[k] <= 1.0)) return fail(TASEP_ERR_ARG, "site densities must lie in [0, 1]");
+        a.p[k] = p[k];
+    }
+    c->cur = 0;
+    if (c->prec == TASEP_F32)
+        hipLaunchKernelGGL(k_product_state<float>, dim3(grid_for(c->dim)), dim3(kBlock), 0, c->stream,
+                           (float *)xcur(c), c->dim, c->n, a);
+    else
+        hipLaunchKernelGGL(k_product_state<double>, dim3(grid_for(c->dim)), dim3(kBlock), 0, c->stream,
+                           (double *)xcur(c), c->dim, c->n, a);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return TASEP_OK;
+}
+
+int tasep_get_x_range(tasep_ctx *c, uint64_t offset, uint64_t count, double *x) {
+    if (!c || !x) return fail(TASEP_ERR_ARG, "NULL argument");
+    if (offset > c->dim || count > c->dim - offset) return fail(TASEP_ERR_ARG, "range outside the state vector");
+    if (!count) return TASEP_OK;
+    if (c->prec == TASEP_F64) {
+        HIP_OK(hipMemcpyAsync(x, (const double *)xcur(c) + offset, count * 8, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        double *tmp = nullptr;
+        HIP_OK(hipMallocAsync((void **)&tmp, count * 8, c->stream));
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(count)), dim3(kBlock), 0, c->stream,
+                           (const float *)xcur(c) + offset, tmp, count);
+        HIP_OK(hipMemcpyAsync(x, tmp, count * 8, hipMemcpyDeviceToHost, c->stream));
         HIP_OK(hipFreeAsync(tmp, c->stream));
     }
     HIP_OK(hipStreamSynchronize(c->stream));
@@ -418,10 +441,10 @@ static void rk4_horner_step(tasep_ctx *c, double h) {
     const double e = (double)sizeof(V);
     const double csr = CSR ? 8.0 + 16.0 * (2.0 + (c->n - 1) / 4.0) : 0.0;
     // P1: u = x + h/4 L x          (read x, write u)
-    a.gx = x; a.gx_is_x = 1; a.out = u; a.c = 0.25 * h;
+    a.gx = x; a.out = u; a.c = 0.25 * h;
     launch_pass<V, CSR, false, EPI_STORE, true>(c, a, "horner_p1", 2 * e + csr);
     // P2: v = x + h/3 L u          (read u, x, write v)
-    a.gx = u; a.gx_is_x = 0; a.out = v; a.c = h / 3.0;
+    a.gx = u; a.out = v; a.c = h / 3.0;
     launch_pass<V, CSR, false, EPI_STORE>(c, a, "horner_p23", 3 * e + csr);
     // P3: u = x + h/2 L v
     a.gx = v; a.out = u; a.c = 0.5 * h;
@@ -440,7 +463,6 @@ static void rk4_staged_step(tasep_ctx *c, double h, uint64_t j) {
     PassArgs<V> a = pass_base<V>(c);
     a.x = x;
     a.gx = x;
-    a.gx_is_x = 1;
     a.c = h;
     a.use_self = 0;
     const double e = (double)sizeof(V);
@@ -710,10 +732,9 @@ static int prepare_run(tasep_ctx *c, double t0, double h, double thr, int patien
             if (rc) return rc;
         }
     }
-    // fresh loop state; an FP32 vector keeps the scale of its stored mass
+    // fresh loop state, or the previous run's with new step parameters
     HIP_OK(hipMemcpyAsync(c->st_host, c->st, sizeof(RunState), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
-    const double scale = c->st_host->scale;
     c->steps_before = resume ? c->st_host->steps_done : 0;
     if (resume) {
         c->st_host->h = h;
@@ -729,12 +750,6 @@ static int prepare_run(tasep_ctx *c, double t0, double h, double thr, int patien
     c->st_host->h = h;
     c->st_host->thr_h = thr * h;
     c->st_host->patience = c->tv ? -1 : patience;
-    // The FP32 vector is never rescaled in place (a factor within 1e-9 of 1
-    // is below its resolution); its FP64 mass is tracked every step and
-    // tasep_moments divides by it.
-    c->st_host->renorm = 0;
-    c->st_host->scale = 1.0;
-    (void)scale;
     HIP_OK(hipMemcpyAsync(c->st, c->st_host, sizeof(RunState), hipMemcpyHostToDevice, c->stream));
     return TASEP_OK;
 }

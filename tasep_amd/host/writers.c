@@ -11,8 +11,9 @@
  * The post-processing at the end (error guess, closures, point functions,
  * the high-order correlation, src/masterEquation.c:32-143, 1422-1539) sums
  * 4-site window marginals computed on the device instead of looping over
- * all 2^n states per value; the timed correlation of -a snapshots (6-site
- * keys) still loops over the snapshot states.
+ * all 2^n states per value; the timed correlation of -a snapshots (4-site
+ * keys whose sums span 6 sites) sums the 6-site window marginals each
+ * snapshot keeps.
  */
 #include "writers.h"
 

@@ -7,7 +7,7 @@ matgen.c, solv.c, matvec.c, io.c, graphlib.c built by gcc from where they
 lie).  The fixtures written here are data only: inputs and the outputs the
 reference produced for them.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 
 Outputs
   csr_n{n}_{tag}.npz     rowPtr, colInd, val from the reference's gencrs
@@ -132,6 +132,25 @@ def cli_cases():
                              "amp.txt": rng.uniform(0, 0.5, n + 1),
                              "per.txt": rng.uniform(0.5, 2.0, n + 1),
                              "const.txt": np.ones(n + 1)}))
+    # -a snapshots (every 10th step) with the default selection (e, p, c, m):
+    # timed expected values, snapshot probabilities, the covariance and
+    # multiplicative expected-value directories with per-snapshot PPMs.
+    r6a = rates_seeded(6, 66)
+    for tag, extra in (("f", ["-f"]), ("mf", [])):
+        cases.append(dict(name=f"n6_anim_{tag}", n=6,
+                          argv=["-n", "6", "-i", "rates.txt", "-t", "0", "-T", "1",
+                                "-s", "100", "-a", "10"] + extra + ["-o", "out"],
+                          files={"rates.txt": r6a}))
+    rng = np.random.default_rng(57)
+    cases.append(dict(name="n5_anim_tv", n=5,
+                      argv=["-n", "5", "-i", "rates.txt", "--time-var",
+                            "--amplitude", "amp.txt", "--period", "per.txt",
+                            "--constant", "const.txt", "-t", "0", "-T", "2",
+                            "-s", "200", "-a", "25", "-o", "out"],
+                      files={"rates.txt": rates_seeded(5, 58),
+                             "amp.txt": rng.uniform(0, 0.5, n + 1),
+                             "per.txt": rng.uniform(0.5, 2.0, n + 1),
+                             "const.txt": np.ones(n + 1)}))
     return cases
 
 
@@ -171,9 +190,12 @@ def run_cli_case(case):
 def main():
     if not oracle.have_ref() or not os.path.exists(oracle.REF_BIN_PATH):
         sys.exit("oracle/_ref missing: run `make -C oracle ref` first")
-    write_csr_fixtures()
+    only = set(sys.argv[1:])        # names of CLI cases to (re)write; none: all
+    if not only:
+        write_csr_fixtures()
     for case in cli_cases():
-        run_cli_case(case)
+        if not only or case["name"] in only:
+            run_cli_case(case)
 
 
 if __name__ == "__main__":

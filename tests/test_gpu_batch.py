@@ -192,3 +192,41 @@ def test_batch_errors():
             b.set_option(99, 1)
         with pytest.raises(T.TasepError):
             b.set_rates(2, np.ones(9))   # no such instance
+
+
+def test_config4_workload_matches_oracle_per_instance():
+    """The benchmark batch itself -- 512 instances, n ~ U{8..18}, rates
+    ~ LogNormal(0, 0.5), odd instances time-variant (seed 512) -- under the
+    autotuned schedule with its XCD block table and graph replay: every
+    instance against its own FP64 oracle RK4 run after 20 steps (FP32
+    state, <= 1e-6), every instance's steps, time and FP64 mass."""
+    import bench
+    ns, spec = bench.config_4_instances()
+    assert len(ns) == 512 and min(ns) == 8 and max(ns) == 18
+    steps, h = 20, 0.01
+    with T.Batch(ns, "f32") as b:
+        for i, (kind, val) in enumerate(spec):
+            if kind == "tv":
+                b.set_time_variant(i, val)
+            else:
+                b.set_rates(i, val)
+        infos = b.run(0.0, h, steps)
+        chosen = b.get_option(T.BATCH_OPT_SCHEDULE)
+        xs = [b.get_x(i) for i in range(len(ns))]
+    print(f"config-4 batch: schedule {chosen}")
+    assert chosen in (1, 2)
+    worst = 0.0
+    for i, (n, (kind, val)) in enumerate(zip(ns, spec)):
+        x0 = oracle.uniform_x0(n)
+        if kind == "tv":
+            ox, _ = oracle.run_rk4(n, None, x0, 0.0, h, steps, tv=val)
+        else:
+            ox, _ = oracle.run_rk4(n, val, x0, 0.0, h, steps)
+        err = float(np.max(np.abs(xs[i] - ox)))
+        worst = max(worst, err)
+        assert err <= 1e-6, (i, n, kind, err)
+        assert infos[i].steps_done == steps and not infos[i].stopped
+        assert infos[i].t == pytest.approx(h * steps, abs=1e-12)
+        assert abs(infos[i].mass - 1.0) <= 1e-5, (i, infos[i].mass)
+        assert abs(xs[i].sum() - 1.0) <= 1e-5
+    print(f"config-4 batch: max over instances of max|x - x_oracle| = {worst:.3e}")

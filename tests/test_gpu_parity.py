@@ -393,6 +393,217 @@ def test_ref_loop_largest_fp64_lattice_n33_uniform_is_stationary():
     assert abs(mass - 1.0) <= 1e-12
 
 
+def _product_np(n, p):
+    """numpy twin of tasep_set_x0_product: prod over bits k ascending of
+    (b_k ? p[k] : 1 - p[k]), the same FP64 operations in the same order."""
+    s = np.arange(1 << n, dtype=np.uint64)
+    x = np.ones(1 << n)
+    for k in range(n):
+        b = ((s >> np.uint64(k)) & np.uint64(1)).astype(bool)
+        x *= np.where(b, p[k], 1.0 - p[k])
+    return x
+
+
+def _site_densities(n, seed):
+    """Distinct densities per bit (the state's value depends on every bit)."""
+    return np.random.default_rng(seed).uniform(0.15, 0.85, n)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_product_state_and_range_read(prec):
+    n = 12
+    p = _site_densities(n, 3)
+    want = _product_np(n, p)
+    with T.Engine(n, prec, "rk4", "mf") as e:
+        e.set_x0_product(p)
+        x = e.get_x()
+        part = e.get_x_range(1000, 77)
+        assert e.get_x_range(4095, 1).shape == (1,)
+        with pytest.raises(T.TasepError):
+            e.get_x_range(4090, 7)
+        with pytest.raises(T.TasepError):
+            e.set_x0_product(np.full(n, 1.5))
+    if prec == "f64":
+        assert _bits_equal(x, want)
+    else:
+        assert np.array_equal(x, want.astype(np.float32).astype(np.float64))
+    assert np.array_equal(part, x[1000:1077])
+
+
+def _euler_densities(q, r, h):
+    """Site densities (entry .. exit) after one explicit Euler step from the
+    product measure with site densities q: rho_j' = q_j + h (J_{j-1} - J_j),
+    J_0 = alpha (1 - q_1), J_j = r_j q_j (1 - q_{j+1}), J_n = beta q_n
+    (the product measure factorises every pair moment)."""
+    n = len(q)
+    J = np.empty(n + 1)
+    J[0] = r[0] * (1 - q[0])
+    J[n] = r[n] * q[n - 1]
+    for j in range(1, n):
+        J[j] = r[j] * q[j - 1] * (1 - q[j])
+    return q + h * (J[:-1] - J[1:])
+
+
+def test_ref_loop_n33_product_state_one_step_closed_form():
+    """The reference loop (an Euler step, four equal stages) on FP64 tiles
+    at n=33 (64 GiB per vector, 2^21 tiles, the 16384-workgroup grid) from a
+    product state whose value depends on every bit, with random rates: the
+    densities after one step equal the closed form to rounding, and the
+    mass stays 1.  A partner read from a wrong address (a truncated index, a
+    wrong tile, chunk or lane) changes some density by about h times the
+    affected mass."""
+    n, h = 33, 0.01
+    pb = _site_densities(n, 33)
+    q = pb[::-1]                                    # site densities entry .. exit
+    r = np.random.default_rng(34).uniform(0.2, 1.8, n + 1)
+    with T.Engine(n, "f64", "ref", "mf") as e:
+        e.set_rates(r)
+        e.set_x0_product(pb)
+        y0, _, _, m0 = e.moments()
+        info = e.run(0.0, h, 1, patience=-1)
+        y, _, _, mass = e.moments()
+    assert np.max(np.abs(y0 - pb)) <= 1e-13 and abs(m0 - 1.0) <= 1e-13
+    want = _euler_densities(q, r, h)
+    err = np.max(np.abs(y[::-1] - want))
+    print(f"n=33 one reference step: max |rho - closed form| = {err:.3e}, |mass - 1| = {abs(mass - 1):.2e}")
+    assert info.steps_done == 1
+    assert err <= 1e-13
+    assert abs(mass - 1.0) <= 1e-13
+
+
+def _slices(dim, count=24, width=1 << 14, seed=0):
+    """Offsets of state slices spread over the whole vector: both ends, the
+    half and quarter points (the top bits), and random positions."""
+    rng = np.random.default_rng(seed)
+    offs = {0, dim - width, dim // 2 - width // 2, dim // 4, 3 * dim // 4, dim // 8 - width}
+    while len(offs) < count:
+        offs.add(int(rng.integers(0, dim - width)))
+    return sorted(offs), width
+
+
+def _rk4_product_run(n, prec, r, pb, steps, slices, width, **opts):
+    with T.Engine(n, prec, "rk4", "mf") as e:
+        for k, v in opts.items():
+            e.set_option(getattr(T, "OPT_" + k.upper()), int(v))
+        e.set_rates(r)
+        e.set_x0_product(pb)
+        info = e.run(0.0, 0.01, steps, patience=-1)
+        y, cov, _, mass = e.moments()
+        W = e.window_marginals()
+        xs = [e.get_x_range(o, width) for o in slices]
+    assert info.steps_done == steps
+    return y, cov, mass, W, xs
+
+
+def test_config5_n32_rk4_f64_tiles_vs_per_state_kernel_from_product_state():
+    """Config 5 (n=32, alpha = 0.3, beta = 0.7, gamma = 1) in FP64, from a
+    product state whose value depends on every bit: two RK4 steps on the
+    tiles (16384 workgroups, per-XCD sweep) against the one-lane-per-state
+    kernel, which derives every neighbour from the bitmask of the global
+    index.  Compared: moments and 4-site window marginals over all 2^32
+    states, and 24 slices of 2^14 states element by element."""
+    n = 32
+    r = np.array([0.3] + [1.0] * (n - 1) + [0.7])
+    pb = _site_densities(n, 32)
+    offs, width = _slices(1 << n)
+    a = _rk4_product_run(n, "f64", r, pb, 2, offs, width)
+    b = _rk4_product_run(n, "f64", r, pb, 2, offs, width, tiled=0)
+    assert np.max(np.abs(a[0] - b[0])) <= 1e-13
+    assert np.max(np.abs(a[1] - b[1])) <= 1e-13
+    assert abs(a[2] - b[2]) <= 1e-13
+    assert np.max(np.abs(a[3] - b[3])) <= 1e-13
+    worst = max(float(np.max(np.abs(u - v) / np.abs(v))) for u, v in zip(a[4], b[4]))
+    print(f"n=32 FP64 tiles vs one lane: worst relative element difference {worst:.2e}")
+    assert worst <= 1e-12
+
+
+def test_config5_n32_rk4_f32_tiles_vs_f64_tiles_from_product_state():
+    """The config-5 FP32 run (n=32, 2^32 FP32 states) against the FP64 tiles
+    (themselves checked against the one-lane kernel above) from the same
+    product state, two steps: densities, covariances and window marginals
+    within 1e-6, every element of 24 slices within 1e-5 relative."""
+    n = 32
+    r = np.array([0.3] + [1.0] * (n - 1) + [0.7])
+    pb = _site_densities(n, 32)
+    offs, width = _slices(1 << n)
+    a = _rk4_product_run(n, "f32", r, pb, 2, offs, width)
+    b = _rk4_product_run(n, "f64", r, pb, 2, offs, width)
+    assert np.max(np.abs(a[0] - b[0])) <= 1e-6
+    assert np.max(np.abs(a[1] - b[1])) <= 1e-6
+    assert abs(a[2] - b[2]) <= 1e-5
+    assert np.max(np.abs(a[3] - b[3])) <= 1e-6
+    worst = max(float(np.max(np.abs(u - v) / np.abs(v))) for u, v in zip(a[4], b[4]))
+    print(f"n=32 FP32 vs FP64 tiles: worst relative element difference {worst:.2e}")
+    assert worst <= 1e-5
+
+
+@pytest.mark.parametrize("n,steps", [(20, 30), (22, 12)])
+def test_tiled_f32_vs_oracle_from_product_state(n, steps):
+    """FP32 tiled passes at n = 20, 22 (several tile bonds per tile, the
+    reads beyond the pre-issued two) against the FP64 oracle, from a product
+    state whose value depends on every bit: every element within 1e-4
+    relative (a wrong partner moves an element by about h r, 1e-2
+    relative)."""
+    r = np.random.default_rng(60 + n).uniform(0.2, 1.5, n + 1)
+    pb = _site_densities(n, 61 + n)
+    x0 = _product_np(n, pb)
+    ox, _ = oracle.run_rk4(n, r, x0, 0.0, 0.01, steps)
+    with T.Engine(n, "f32", "rk4", "mf") as e:
+        e.set_rates(r)
+        e.set_x0_product(pb)
+        e.run(0.0, 0.01, steps, patience=-1)
+        x = e.get_x()
+    rel = float(np.max(np.abs(x - ox) / np.abs(ox)))
+    print(f"n={n} FP32 tiles vs oracle: worst relative element error {rel:.2e}")
+    assert rel <= 1e-4
+
+
+def _random_state(n, seed):
+    x = np.random.default_rng(seed).random(1 << n)
+    return x / x.sum()
+
+
+def test_grid_16384_tile_orders_bitwise_n27_ref_loop():
+    """The 16384-workgroup grid (two FP64 tiles per workgroup at n=27) in
+    every tile order, from a random normalised state: the tiled reference
+    loop equals the one-lane kernel bit for bit."""
+    n = 27
+    r = np.random.default_rng(27).uniform(0.2, 1.8, n + 1)
+    x0 = _random_state(n, 270)
+    outs = []
+    for opts in [dict(tiled=0)] + [dict(tiled=1, tile_blocks=16384, xcd_order=o) for o in range(4)]:
+        with T.Engine(n, "f64", "ref", "mf") as e:
+            for k, v in opts.items():
+                e.set_option(getattr(T, "OPT_" + k.upper()), v)
+            e.set_rates(r)
+            e.set_x(x0)
+            e.run(0.0, 0.01, 2, patience=-1)
+            outs.append(e.get_x())
+    for o in outs[1:]:
+        assert _bits_equal(o, outs[0])
+
+
+def test_grid_16384_tile_orders_n27_rk4_f64():
+    """The same grid and orders for the RK4 tiles (FP64 Horner): every
+    order gives the same bits, and the tiles agree with the one-lane kernel
+    to 1e-13 relative per element."""
+    n = 27
+    r = np.random.default_rng(28).uniform(0.2, 1.8, n + 1)
+    x0 = _random_state(n, 271)
+    outs = []
+    for opts in [dict(tiled=0)] + [dict(tiled=1, tile_blocks=16384, xcd_order=o) for o in range(4)]:
+        with T.Engine(n, "f64", "rk4", "mf") as e:
+            for k, v in opts.items():
+                e.set_option(getattr(T, "OPT_" + k.upper()), v)
+            e.set_rates(r)
+            e.set_x(x0)
+            e.run(0.0, 0.01, 2, patience=-1)
+            outs.append(e.get_x())
+    for o in outs[2:]:
+        assert _bits_equal(o, outs[1])
+    assert float(np.max(np.abs(outs[1] - outs[0]) / np.abs(outs[0]))) <= 1e-13
+
+
 def test_full_size_n28_ref_tiles_bitwise_vs_per_state_kernel():
     """The bit-exact reference loop at the n=28 benchmark size: the tiled
     kernel (per-XCD sweep, pre-issued streams, hoisted diagonal, division by
@@ -425,6 +636,65 @@ def test_converged_rk4_matches_dehp():
     assert info.stopped
     rho = np.array([float(v) for v in dehp.densities(n, a, b)])
     assert np.max(np.abs(y[::-1] - rho)) < 1e-7
+
+
+def test_converged_pair_moments_match_dehp_two_point():
+    """Converged FP64 RK4 pair moments <tau_i tau_j> (all i < j, n=12)
+    against the exact two-point function (steadystates_gmp.c:188-326 with
+    the Z index of the formula; pinned by brute force on the CPU)."""
+    from oracle import dehp
+    n = 12
+    a, b = 0.35, 0.6
+    r = np.array([a] + [1.0] * (n - 1) + [b])
+    with T.Engine(n, "f64", "rk4", "mf") as e:
+        e.set_rates(r)
+        e.set_x0_uniform()
+        info = e.run(0.0, 0.05, 40000, thr=1e-11, patience=100)
+        y, cov, mul, mass = e.moments()
+    assert info.stopped
+    assert abs(mass - 1.0) < 1e-12
+    err = 0.0
+    for i in range(1, n + 1):
+        for j in range(i + 1, n + 1):
+            want = float(dehp.two_point(i, j, a, b, n))
+            err = max(err, abs(mul[n - i, n - j] - want), abs(mul[n - j, n - i] - want))
+    print(f"n={n} max |<tau_i tau_j> - DEHP| = {err:.3e}")
+    assert err < 1e-8
+
+
+def _config2_twin():
+    """Homogeneous twin of the n=20 configuration with rates ~ U[0.1, 2]
+    (seed 20): its alpha and beta with every bulk rate 1 (alpha = 0.632,
+    beta = 0.989, maximal-current phase)."""
+    n = 20
+    r = np.random.default_rng(20).uniform(0.1, 2.0, n + 1)
+    return n, float(r[0]), float(r[n])
+
+
+@pytest.mark.parametrize("integ,prec,h", [("ref", "f64", 0.05), ("rk4", "f64", 0.1)])
+def test_config2_twin_stationary_current_matches_dehp(integ, prec, h):
+    """n=20 homogeneous twin run to convergence (exit test at 1e-12 h): the
+    per-site densities and the stationary current, taken at the entry, at the
+    exit and on every bulk bond (rho_j - <tau_j tau_{j+1}>), equal the exact
+    DEHP steady state."""
+    from oracle import dehp
+    n, a, b = _config2_twin()
+    r = np.array([a] + [1.0] * (n - 1) + [b])
+    with T.Engine(n, prec, integ, "mf") as e:
+        e.set_rates(r)
+        e.set_x0_uniform()
+        info = e.run(0.0, h, 400000, thr=1e-12, patience=100)
+        y, cov, mul, mass = e.moments()
+    print(f"{integ}: stopped after {info.steps_done} steps, t = {info.t:.1f}")
+    assert info.stopped
+    rho = np.array([float(v) for v in dehp.densities(n, a, b)])
+    site = y[::-1] / mass                                 # entry .. exit
+    assert np.max(np.abs(site - rho)) < 1e-8
+    J = float(dehp.current(n, a, b))
+    Js = [a * (1 - site[0]), b * site[-1]]
+    Js += [site[j - 1] - mul[n - j, n - j - 1] / mass for j in range(1, n)]
+    print(f"DEHP J = {J:.12f}, max |J_k - J| = {max(abs(v - J) for v in Js):.3e}")
+    assert max(abs(v - J) for v in Js) < 1e-8
 
 
 # ----------------------------------------------------- tiled kernel coverage

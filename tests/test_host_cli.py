@@ -106,6 +106,47 @@ def test_host_matches_reference_files(name, tmp_path):
     sim = [ln for ln in p.stdout.splitlines() if ln.startswith("Simulation time elapsed: ")]
     assert sim and sim[-1][len("Simulation time elapsed:"):].strip() == \
         G.stdout_value(c, "Simulation time elapsed:")
+    if "-a" in c["argv"]:
+        _check_snapshot_outputs(c, tmp_path)
+
+
+def _strip_undefined(block):
+    """A snapshot matrix block without its last number: the (row 0, col 0)
+    entry the reference sums onto an uninitialised accumulator
+    (src/lib/matvec.c:93-105), written last (src/masterEquation.c:1263-1271)."""
+    body = block.rstrip("\n")
+    return body[:body.rindex(", ") + 2]
+
+
+def _check_snapshot_outputs(c, out_dir):
+    """-a outputs (src/masterEquation.c:1041-1060, 1097-1116, 1197-1279,
+    1352-1414): timed expected values and snapshot probabilities
+    byte-equal; the snapshot covariance and multiplicative expected-value
+    matrices text-equal and their PPMs byte-equal apart from the one
+    undefined entry per snapshot; the final PPMs byte-equal; the timed
+    high-order correlation in layout (its values come from a function
+    without a return value, src/masterEquation.c:47-75)."""
+    assert open(out_dir / "out_exp_timed.txt", "rb").read() == G.read_bytes(c, "out_exp_timed.txt")
+    for d in ("out_covariance_data", "out_multExpval_data"):
+        got_files = sorted(os.listdir(out_dir / d))
+        assert got_files == sorted(os.listdir(os.path.join(c["dir"], d))), d
+        a = [b for b in open(out_dir / d / "out_all.txt").read().split("\n\n\n") if b]
+        b = [b for b in G.read_text(c, d + "/out_all.txt").split("\n\n\n") if b]
+        assert len(a) == len(b) and len(a) > 1
+        for u, v in zip(a, b):
+            assert _strip_undefined(u) == _strip_undefined(v), d
+        for f in got_files:
+            if not f.endswith(".ppm"):
+                continue
+            u = open(out_dir / d / f, "rb").read()
+            v = G.read_bytes(c, d + "/" + f)
+            if f == "out_fin.ppm":
+                assert u == v, (d, f)
+            else:
+                assert len(u) == len(v) and u[:-3] == v[:-3], (d, f)
+    a = [ln.split() for ln in open(out_dir / "out_highcorr_timed").read().splitlines()]
+    b = [ln.split() for ln in G.read_text(c, "out_highcorr_timed").splitlines()]
+    assert [(r[0], len(r)) for r in a] == [(r[0], len(r)) for r in b]
 
 
 @needs_bin

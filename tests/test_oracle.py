@@ -437,3 +437,133 @@ def test_rk4_time_variant_oracle_is_fourth_order():
                   (0.5, 0.5, 0.5, 1.0)):
         e = [float(np.max(np.abs(_rk4_dense(n, tv, x0, T, s, wrong) - exact))) for s in steps]
         assert max(_order_ratios(e)) < 4, (wrong, e)
+
+
+# ------------------------------------------------ DEHP two-point function (f4)
+def _pairs_from_p(n, p):
+    """<tau_i tau_j> (sites 1-based from the entry) of a distribution indexed
+    by bitmask; site i is bit n - i."""
+    return {(i, j): sum(p[s] for s in range(1 << n) if (s >> (n - i)) & 1 and (s >> (n - j)) & 1)
+            for i in range(1, n + 1) for j in range(i + 1, n + 1)}
+
+
+def test_two_point_matches_exact_bruteforce():
+    """oracle.dehp.two_point (steadystates_gmp.c:188-326 with Z_m in place of
+    calculate_Z_i's Z_{m-1}, plus the exit-site case j = N) equals the pair
+    moments of the exact null vector of the generator."""
+    from fractions import Fraction as F
+    from oracle import dehp
+    for n in (2, 3, 4, 5, 6):
+        for a, b in ((F(1, 3), F(3, 4)), (F(3, 2), F(1, 4)), (F(6, 10), F(9, 10))):
+            p = dehp.steady_state_bruteforce(n, [a] + [1] * (n - 1) + [b])
+            for (i, j), want in _pairs_from_p(n, p).items():
+                assert dehp.two_point(i, j, a, b, n) == want, (n, a, b, i, j)
+    # gamma-rescaled twin
+    n, a, b, g = 4, F(6, 10), F(8, 10), F(2)
+    p = dehp.steady_state_bruteforce(n, [a] + [g] * (n - 1) + [b])
+    for (i, j), want in _pairs_from_p(n, p).items():
+        assert dehp.two_point(i, j, a, b, n, g) == want
+
+
+def test_two_point_current_identity():
+    """Bulk current J = <tau_j (1 - tau_{j+1})> = rho_j - <tau_j tau_{j+1}> is
+    the same on every bond and equals alpha (1 - rho_1) = beta rho_N (a check
+    at sizes the brute force cannot reach)."""
+    from fractions import Fraction as F
+    from oracle import dehp
+    for n, a, b in ((10, F(1, 3), F(3, 4)), (12, F(7, 5), F(2, 5)), (9, F(1, 2), F(1, 2))):
+        rho = dehp.densities(n, a, b)
+        J = dehp.current(n, a, b)
+        assert J == b * rho[-1]
+        for j in range(1, n):
+            assert rho[j - 1] - dehp.two_point(j, j + 1, a, b, n) == J
+
+
+def test_two_point_reference_literal_restatement():
+    """The reference's own two-point sums (Z through calculate_Z_i, one index
+    low): with alpha + beta = 1 the stationary state is the Bernoulli(alpha)
+    product and the Z are geometric, so the literal value is alpha^2 like the
+    exact one; elsewhere it is not the pair moment."""
+    from fractions import Fraction as F
+    from oracle import dehp
+    a, b = F(3, 10), F(7, 10)
+    for n in (4, 6, 9):
+        for i in range(1, n - 1):
+            for j in range(i + 1, n):
+                assert dehp.two_point_reference(i, j, a, b, n) == a * a
+                assert dehp.two_point(i, j, a, b, n) == a * a
+    a, b = F(1, 3), F(3, 4)
+    assert dehp.two_point_reference(2, 4, a, b, 6) != dehp.two_point(2, 4, a, b, 6)
+    with pytest.raises(ValueError):
+        dehp.two_point_reference(1, 6, a, b, 6)
+
+
+# ------------------------------------------------------ -a snapshots (f2)
+def _oracle_snapshots(c):
+    """States at the reference's -a snapshots (src/masterEquation.c:646-666,
+    :899-920, :950-966): after the update of every step index i with
+    i % res == 0, i.e. after steps 1, res+1, 2 res+1, ... of the run."""
+    n = c["n"]
+    res = int(c["argv"][c["argv"].index("-a") + 1])
+    h = oracle.step_h(c["T"], c["t0"], c["steps"])
+    x = oracle.uniform_x0(n)
+    t = c["t0"]
+    snaps = []
+    done = 0
+    while done < c["steps"]:
+        k = 1 if done == 0 else min(res, c["steps"] - done)
+        if c["tv"] is not None:
+            x, info = oracle.run_ref_tv(n, x, t, h, k, c["tv"])
+        else:
+            x, info = oracle.run_ref(n, c["rates"], x, t, h, k, patience=-1, use_csr=c["csr"])
+        t = info.t
+        if (done + k - 1) % res == 0:          # index of the last step run
+            snaps.append(x.copy())
+        done += k
+    return snaps, h, res
+
+
+def _matrix_blocks(snaps, n, which):
+    out = []
+    for j, x in enumerate(snaps):
+        cov, mul = oracle.cov(n, x)
+        m = cov if which == "cov" else mul
+        rows = [", ".join("%f" % v for v in row) for row in m]
+        out.append("%d Matrix:\n" % j + "\n".join(rows) + "\n\n\n")
+    return out
+
+
+ANIM = [nm for nm in G.cli_case_names() if "-a" in G.cli_case(nm)["argv"]]
+
+
+def _strip_undefined(block):
+    """Drop the last number of a matrix block: the snapshot covariance's
+    (row 0, col 0) entry, summed onto an uninitialised accumulator by the
+    reference (src/lib/matvec.c:93-105) and written last (:1263-1271)."""
+    body = block.rstrip("\n")
+    return body[:body.rindex(", ") + 2]
+
+
+@pytest.mark.parametrize("name", ANIM)
+def test_snapshot_files_vs_reference_cli(name):
+    """The oracle's states at the -a snapshots reproduce the reference's
+    timed expected values, snapshot probabilities and the covariance and
+    multiplicative expected-value matrices of every snapshot (text-equal,
+    apart from the one undefined entry per snapshot matrix)."""
+    c = G.cli_case(name)
+    n = c["n"]
+    snaps, h, res = _oracle_snapshots(c)
+    assert len(snaps) == -(-c["steps"] // res)
+    times = ["%f" % (h * res * j) for j in range(len(snaps))]
+    exp = "".join(times[j] + ", " + ", ".join("%f" % v for v in oracle.site_moments(n, x)[::-1]) + "\n"
+                  for j, x in enumerate(snaps))
+    assert exp == G.read_text(c, "out_exp_timed.txt")
+    prb = "".join(times[j] + ", " + ", ".join("%f" % v for v in x) + "\n" for j, x in enumerate(snaps))
+    assert prb == G.read_text(c, "out_prb.txt")
+    for d, which in (("out_covariance_data", "cov"), ("out_multExpval_data", "mul")):
+        ref = G.read_text(c, d + "/out_all.txt")
+        want = _matrix_blocks(snaps, n, which)
+        got_blocks = [b for b in ref.split("\n\n\n") if b]
+        assert len(got_blocks) == len(want)
+        for a, b in zip(got_blocks, want):
+            assert _strip_undefined(a) == _strip_undefined(b)
