@@ -118,6 +118,20 @@ def _strip_undefined(block):
     return body[:body.rindex(", ") + 2]
 
 
+def _numbers(text):
+    return np.array([float(v) for v in text.replace(",", " ").replace("Matrix:", " ").split()])
+
+
+def _same_text(a, b, exact):
+    """Text-equal; for time-variant runs (state within 1e-12 of the
+    reference's: device sin against glibc's) the printed %lf values agree
+    to their last digit, a zero's sign aside."""
+    if exact:
+        return a == b
+    u, v = _numbers(a), _numbers(b)
+    return u.shape == v.shape and float(np.max(np.abs(u - v), initial=0.0)) <= 1.5e-6
+
+
 def _check_snapshot_outputs(c, out_dir):
     """-a outputs (src/masterEquation.c:1041-1060, 1097-1116, 1197-1279,
     1352-1414): timed expected values and snapshot probabilities
@@ -125,8 +139,10 @@ def _check_snapshot_outputs(c, out_dir):
     matrices text-equal and their PPMs byte-equal apart from the one
     undefined entry per snapshot; the final PPMs byte-equal; the timed
     high-order correlation in layout (its values come from a function
-    without a return value, src/masterEquation.c:47-75)."""
-    assert open(out_dir / "out_exp_timed.txt", "rb").read() == G.read_bytes(c, "out_exp_timed.txt")
+    without a return value, src/masterEquation.c:47-75).  Time-variant runs:
+    values to the printed digit, PPM bytes within 1."""
+    exact = c["tv"] is None
+    assert _same_text(open(out_dir / "out_exp_timed.txt").read(), G.read_text(c, "out_exp_timed.txt"), exact)
     for d in ("out_covariance_data", "out_multExpval_data"):
         got_files = sorted(os.listdir(out_dir / d))
         assert got_files == sorted(os.listdir(os.path.join(c["dir"], d))), d
@@ -134,16 +150,20 @@ def _check_snapshot_outputs(c, out_dir):
         b = [b for b in G.read_text(c, d + "/out_all.txt").split("\n\n\n") if b]
         assert len(a) == len(b) and len(a) > 1
         for u, v in zip(a, b):
-            assert _strip_undefined(u) == _strip_undefined(v), d
+            assert _same_text(_strip_undefined(u), _strip_undefined(v), exact), d
         for f in got_files:
             if not f.endswith(".ppm"):
                 continue
             u = open(out_dir / d / f, "rb").read()
             v = G.read_bytes(c, d + "/" + f)
-            if f == "out_fin.ppm":
+            if f != "out_fin.ppm":        # the undefined entry's pixel comes last
+                u, v = u[:-3], v[:-3]
+            assert len(u) == len(v), (d, f)
+            if exact:
                 assert u == v, (d, f)
             else:
-                assert len(u) == len(v) and u[:-3] == v[:-3], (d, f)
+                du = np.frombuffer(u, np.uint8).astype(int) - np.frombuffer(v, np.uint8).astype(int)
+                assert np.max(np.abs(du)) <= 1, (d, f)
     a = [ln.split() for ln in open(out_dir / "out_highcorr_timed").read().splitlines()]
     b = [ln.split() for ln in G.read_text(c, "out_highcorr_timed").splitlines()]
     assert [(r[0], len(r)) for r in a] == [(r[0], len(r)) for r in b]
