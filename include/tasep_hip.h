@@ -146,6 +146,21 @@ int tasep_run_resume(tasep_ctx *ctx, double t0, double h, uint64_t steps,
 int tasep_moments(tasep_ctx *ctx, double *y, double *cov, double *mul,
                   double *mass);
 
+/* tasep_moments without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  x is a
+ * caller-owned device vector of 2^n entries (double for TASEP_F64, float for
+ * TASEP_F32), e.g. from tasep_get_x_device; results as tasep_moments (any
+ * output pointer may be NULL), 2 <= n <= 34.  The block partials live in a
+ * static device array of 1.17 MiB per GPU (256 blocks x 596 doubles plus
+ * 596 sums) instead of allocated memory; every call on a GPU shares that
+ * GPU's copy, so calls that overlap on one GPU race.  The work is enqueued
+ * on `stream` (a hipStream_t; NULL = the default stream), which is
+ * synchronized before the call returns. */
+int tasep_moments_noalloc(const void *x, int n, int precision, double *y,
+                          double *cov, double *mul, double *mass,
+                          void *stream);
+
 /* Window marginals of the current state in one device pass with FP64 sums:
  *   W[o*16 + p] = sum_s x_s [((s >> o) & 15) == p],  o = 0..n-4, p = 0..15
  * ((n-3)*16 doubles; n >= 4; FP32 contexts divided by the FP64 mass).

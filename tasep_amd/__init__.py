@@ -36,7 +36,7 @@ EXPORTS = [
     "tasep_batch_get_x", "tasep_batch_run", "tasep_batch_moments",
     "tasep_batch_set_option", "tasep_batch_get_stream", "tasep_window_marginals",
     "tasep_batch_window_marginals", "tasep_batch_get_option", "tasep_window_marginals_w",
-    "tasep_set_x0_product", "tasep_get_x_range",
+    "tasep_set_x0_product", "tasep_get_x_range", "tasep_moments_noalloc",
 ]
 OPT_TILED, OPT_XCD_ORDER, OPT_RK4_STAGED, OPT_TILE_BLOCKS, OPT_LDS, OPT_LDS_THREADS = 1, 2, 3, 4, 5, 6
 OPT_GRAPH, OPT_STORE = 7, 8
@@ -109,6 +109,7 @@ def load_library():
     L.tasep_run_resume.argtypes = [vp, C.c_double, C.c_double, C.c_uint64, C.c_double,
                                    C.c_int, C.c_int, C.POINTER(_RunInfo)]
     L.tasep_moments.argtypes = [vp, vp, vp, vp, vp]
+    L.tasep_moments_noalloc.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
     L.tasep_step_profile.argtypes = [vp, C.c_double, C.c_double, C.c_int,
                                      C.POINTER(_KernelTime), C.c_int,
                                      C.POINTER(C.c_int)]

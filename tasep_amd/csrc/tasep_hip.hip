@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "../../include/tasep_hip.h"
@@ -1015,36 +1016,56 @@ int tasep_get_x_device(tasep_ctx *c, void **ptr) {
 
 }  // extern "C"
 
+// The two moment launches on a state vector x (2^n entries of the given
+// precision): block partials into part (nb x nq doubles, every one written),
+// their fixed-order sums into out (nq doubles), and the sums copied to E.
+static int moment_sums(hipStream_t stream, const void *x, int n, int prec, int max_blocks, double *part,
+                       double *out, double *E) {
+    const uint64_t dim = (uint64_t)1 << n;
+    const int nq = n * (n + 1) / 2 + 1;
+    const uint64_t rows = dim >> kMomRowBits ? dim >> kMomRowBits : 1;
+    const int nb = (int)(rows < (uint64_t)max_blocks ? rows : max_blocks);
+    if (prec == TASEP_F32)
+        hipLaunchKernelGGL(k_moments_partial<float>, dim3(nb), dim3(kBlock), 0, stream, (const float *)x, n, dim,
+                           rows, part);
+    else
+        hipLaunchKernelGGL(k_moments_partial<double>, dim3(nb), dim3(kBlock), 0, stream, (const double *)x, n, dim,
+                           rows, part);
+    hipLaunchKernelGGL(k_moments_final, dim3(nq), dim3(kBlock), 0, stream, part, nb, nq, out);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(E, out, nq * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    return TASEP_OK;
+}
+
+// y, cov, mul, mass from the sums E (the n(n+1)/2 pair sums by tri_index and
+// the mass last); FP32 vectors are divided by their FP64 mass.
+static void moments_from_sums(int n, int prec, double *E, double *y, double *cov, double *mul, double *mass);
+
 // Moments of a state vector x (2^n entries of the given precision) on the
 // device; *part and *out are allocated on first use and kept by the caller.
 int tasep_host::device_moments(hipStream_t stream, const void *x, int n, int prec, double **part, double **out,
                                double *y, double *cov, double *mul, double *mass) {
-    const uint64_t dim = (uint64_t)1 << n;
-    const int nq = n * (n + 1) / 2 + 1;
-    const uint64_t rows = dim >> kMomRowBits ? dim >> kMomRowBits : 1;
-    const int nb = (int)(rows < (uint64_t)kMomMaxBlocks ? rows : kMomMaxBlocks);
     if (!*part) {
         HIP_OK(hipMalloc(part, (size_t)kMomMaxBlocks * kMomMaxQ * sizeof(double)));
         HIP_OK(hipMalloc(out, kWinMaxQ * sizeof(double)));
     }
-    if (prec == TASEP_F32)
-        hipLaunchKernelGGL(k_moments_partial<float>, dim3(nb), dim3(kBlock), 0, stream, (const float *)x, n, dim,
-                           rows, *part);
-    else
-        hipLaunchKernelGGL(k_moments_partial<double>, dim3(nb), dim3(kBlock), 0, stream, (const double *)x, n, dim,
-                           rows, *part);
-    hipLaunchKernelGGL(k_moments_final, dim3(nq), dim3(kBlock), 0, stream, *part, nb, nq, *out);
-    HIP_OK(hipGetLastError());
-    std::vector<double> E(nq);
-    HIP_OK(hipMemcpyAsync(E.data(), *out, nq * sizeof(double), hipMemcpyDeviceToHost, stream));
-    HIP_OK(hipStreamSynchronize(stream));
+    double E[kMomMaxQ];
+    int rc = moment_sums(stream, x, n, prec, kMomMaxBlocks, *part, *out, E);
+    if (rc) return rc;
+    moments_from_sums(n, prec, E, y, cov, mul, mass);
+    return TASEP_OK;
+}
+
+static void moments_from_sums(int n, int prec, double *E, double *y, double *cov, double *mul, double *mass) {
+    const int nq = n * (n + 1) / 2 + 1;
     double M = E[nq - 1];
     if (prec == TASEP_F32 && M > 0.0) {
         for (int q = 0; q < nq - 1; q++) E[q] /= M;
         M = 1.0;
     }
     if (mass) *mass = M;
-    std::vector<double> yy(n);
+    double yy[kMaxN];
     for (int i = 0; i < n; i++) yy[i] = E[tri_index(i, i)];
     if (y) for (int i = 0; i < n; i++) y[i] = yy[i];
     for (int i = 0; i < n; i++) {
@@ -1056,7 +1077,6 @@ int tasep_host::device_moments(hipStream_t stream, const void *x, int n, int pre
             if (mul) mul[i * n + k] = yy[k] * yy[i] + cv;
         }
     }
-    return TASEP_OK;
 }
 
 template <int WW>
@@ -1107,4 +1127,57 @@ extern "C" int tasep_window_marginals_w(tasep_ctx *c, int width, double *W) {
     if (width != kWin && width != kWin6) return fail(TASEP_ERR_ARG, "window width: 4 or 6");
     if (c->n < width) return fail(TASEP_ERR_ARG, "window marginals need n >= width");
     return device_windows(c->stream, xcur(c), c->n, c->prec, &c->mom_part, &c->mom_out, width, W);
+}
+
+// ---------------------------------------------------- allocation-free moments
+// tasep_moments_noalloc keeps its block partials and sums in this file-scope
+// device array instead of allocating (each GPU the library runs on holds its
+// own copy until the process ends).  Host code never uses the array's name
+// as an address: the per-device pointer from hipGetSymbolAddress is what the
+// kernels and copies receive.
+constexpr int kNoAllocBlocks = 256;                                     // partial-sum blocks
+constexpr size_t kNoAllocDoubles = (size_t)kNoAllocBlocks * kMomMaxQ + kMomMaxQ;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];        // 1.17 MiB
+
+constexpr int kMaxDevices = 64;
+static double *g_scratch_dev[kMaxDevices];                              // per device, from hipGetSymbolAddress
+static std::mutex g_scratch_mu;
+
+// The current device's copy of `scratch`, looked up on the first call on
+// that device.
+static int scratch_on_device(double **ptr) {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(TASEP_ERR_NODEV, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(TASEP_ERR_HIP, "hipGetSymbolAddress(scratch)", e);
+        g_scratch_dev[dev] = (double *)p;
+    }
+    *ptr = g_scratch_dev[dev];
+    return TASEP_OK;
+}
+
+extern "C" int tasep_moments_noalloc(const void *x, int n, int precision, double *y, double *cov, double *mul,
+                                     double *mass, void *stream) {
+    if (!x) return fail(TASEP_ERR_ARG, "NULL state vector");
+    if (n < 2 || n > kMaxN) return fail(TASEP_ERR_ARG, "n must be in [2, 34] (the scratch holds 34-site moments)");
+    if (precision != TASEP_F64 && precision != TASEP_F32) return fail(TASEP_ERR_ARG, "unknown precision");
+    hipPointerAttribute_t attr;
+    const hipError_t pe = hipPointerGetAttributes(&attr, x);
+    if (pe != hipSuccess || attr.type != hipMemoryTypeDevice) {
+        (void)hipGetLastError();  // clear the lookup's error state
+        return fail(TASEP_ERR_ARG, "the state vector is not device memory");
+    }
+    double *part = nullptr;
+    int rc = scratch_on_device(&part);
+    if (rc) return rc;
+    double E[kMomMaxQ];
+    rc = moment_sums((hipStream_t)stream, x, n, precision, kNoAllocBlocks, part,
+                     part + (size_t)kNoAllocBlocks * kMomMaxQ, E);
+    if (rc) return rc;
+    moments_from_sums(n, precision, E, y, cov, mul, mass);
+    return TASEP_OK;
 }
