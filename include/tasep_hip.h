@@ -283,15 +283,29 @@ int tasep_batch_window_marginals(tasep_batch *batch, int i, double *W);
  *                                    second stream beside the shared passes
  *   TASEP_BATCH_OPT_SCHEDULE    [0]  shared passes: 1 one lane per state,
  *                                    2 tiles of 2^10 (FP32) / 2^9 (FP64)
- *                                    states in LDS, 0 = timed on the first
- *                                    run and the faster kept (get_option
- *                                    then reports the choice; the two
- *                                    agree to rounding, so pin 1 or 2 for
- *                                    bitwise run-to-run reproducibility) */
+ *                                    states in LDS, 3 the single-lattice
+ *                                    tile pass (tiles of 2^13 / 2^12 states
+ *                                    with the top three bits, a range of
+ *                                    tiles per workgroup; every streamed
+ *                                    instance then has exactly a single
+ *                                    context's tiled results; needs
+ *                                    n >= 14 / 13), 0 = timed on the first
+ *                                    run and the fastest that fits kept
+ *                                    (get_option then reports the choice;
+ *                                    the schedules agree to rounding, so
+ *                                    pin one for bitwise run-to-run
+ *                                    reproducibility)
+ *   TASEP_BATCH_OPT_RANGE_TILES [2]  schedule 3: tiles per workgroup
+ *   TASEP_BATCH_OPT_STORE       [0]  schedule 3 output stores: 0 by the
+ *                                    batch vector's size (ordinary for
+ *                                    16-128 MiB, non-temporal otherwise),
+ *                                    1 non-temporal, 2 ordinary */
 #define TASEP_BATCH_OPT_GRAPH_STEPS 1
 #define TASEP_BATCH_OPT_LDS         2
 #define TASEP_BATCH_OPT_CONCURRENT  3
 #define TASEP_BATCH_OPT_SCHEDULE    4
+#define TASEP_BATCH_OPT_RANGE_TILES 5
+#define TASEP_BATCH_OPT_STORE       6
 int tasep_batch_set_option(tasep_batch *batch, int key, int64_t value);
 int tasep_batch_get_option(tasep_batch *batch, int key, int64_t *value);
 int tasep_batch_get_stream(tasep_batch *batch, void **stream);

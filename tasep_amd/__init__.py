@@ -41,6 +41,7 @@ EXPORTS = [
 OPT_TILED, OPT_XCD_ORDER, OPT_RK4_STAGED, OPT_TILE_BLOCKS, OPT_LDS, OPT_LDS_THREADS = 1, 2, 3, 4, 5, 6
 OPT_GRAPH, OPT_STORE = 7, 8
 BATCH_OPT_GRAPH_STEPS, BATCH_OPT_LDS, BATCH_OPT_CONCURRENT, BATCH_OPT_SCHEDULE = 1, 2, 3, 4
+BATCH_OPT_RANGE_TILES, BATCH_OPT_STORE = 5, 6
 
 
 class TasepError(RuntimeError):

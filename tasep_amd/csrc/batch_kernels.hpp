@@ -20,6 +20,7 @@
 
 #include "device_common.hpp"
 #include "lds_kernels.hpp"
+#include "tile_kernels.hpp"
 
 namespace tasep {
 
@@ -396,6 +397,110 @@ __global__ __launch_bounds__(kBatchTileThreads, 8) void k_batch_tile(const Batch
         if (t == 0) {
             a.pchk[in.part_off + bb.blk] = bc;
             a.pmass[in.part_off + bb.blk] = bm;
+        }
+    }
+}
+
+// ===========================================================================
+// Big-tile form of the batched pass: the single-lattice tile pass
+// (tile_pass_tiles: 512 lanes, tiles of 2^13 FP32 / 2^12 FP64 states with the
+// top three state bits in the tile, every global read of a tile issued before
+// its LDS phase and the next tile's own vector prefetched) over a range of
+// consecutive tiles of one instance per workgroup.  Needs n >= 14 (FP32) or
+// n >= 13 (FP64): two tiles.  Constant-rate instances take the Horner passes,
+// time-variant ones the staged passes with materialised stage inputs (u, v)
+// and the running k-sum in acc, the last of them writing x in place (it
+// gathers u, so x is read only at each state's own slot).  The arithmetic is
+// that of a single context's tiled run, so every instance's state is the
+// same, bit for bit.
+// ===========================================================================
+struct BatchRange {
+    uint32_t inst;   // 0xffffffff: padding, no work
+    uint32_t t0;     // first tile of the range
+    uint32_t count;  // consecutive tiles
+    uint32_t part;   // partial-sum slot of the workgroup
+};
+
+template <class V>
+struct BatchBigArgs {
+    const BatchInst *inst;
+    const BatchRange *ranges;  // one entry per workgroup
+    V *x, *u, *v, *acc;
+    const double *rates;
+    const int32_t *tv_type;
+    const double *tv_amp, *tv_period, *tv_offset, *tv_c;
+    RunState *st;
+    double *pchk, *pmass;
+};
+
+template <class V, int P, bool NT>
+__global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_batch_bigtile(const BatchBigArgs<V> b) {
+    using G = TileGeom<V>;
+    const BatchRange r = b.ranges[blockIdx.x];
+    if (r.inst == 0xffffffffu) return;
+    const BatchInst in = b.inst[r.inst];
+    RunState *st = b.st + in.id;
+    if (st->stopped) return;
+    __shared__ V tile[kTileThreads * G::C * G::E];
+    __shared__ V rr[kRateStride];
+    __shared__ double red[kTileWaves];
+    const int n = in.n, t = threadIdx.x;
+    const double h = st->h;
+    // rates of this pass: constant, or r(t), r(t+h/2), r(t+h) for stages 1, 2-3, 4
+    if (t <= n) {
+        double q;
+        if (in.tv) {
+            const int o = in.tv_off + t;
+            const double t0 = st->t;
+            const double tq = P == 0 ? t0 : (P < 3 ? t0 + 0.5 * h : t0 + h);
+            q = tv_rate(b.tv_type[o], b.tv_amp[o], b.tv_period[o], b.tv_offset[o], b.tv_c[o], tq);
+        } else {
+            q = b.rates[in.rate_off + t];
+        }
+        rr[t] = (V)q;
+    }
+    __syncthreads();
+    TileArgs<V> a = {};
+    a.n = n;
+    a.ntiles = (uint64_t)1 << (n - G::b);
+    a.st = st;
+    a.pchk = b.pchk;
+    a.pmass = b.pmass;
+    a.use_self = 1;
+    V *x = b.x + in.off, *u = b.u + in.off, *v = b.v + in.off;
+    a.x = x;
+    if (!in.tv) {
+        // Horner: u = x + h/4 Lx, v = x + h/3 Lu, u = x + h/2 Lv, x += h Lu
+        if constexpr (P == 0) {
+            a.gx = x; a.out = u; a.c = (V)(0.25 * h);
+            tile_pass_tiles<V, EPI_STORE, true, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+        } else if constexpr (P == 1) {
+            a.gx = u; a.out = v; a.c = (V)(h / 3.0);
+            tile_pass_tiles<V, EPI_STORE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+        } else if constexpr (P == 2) {
+            a.gx = v; a.out = u; a.c = (V)(0.5 * h);
+            tile_pass_tiles<V, EPI_STORE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+        } else {
+            a.gx = u; a.out = x; a.c = (V)h;
+            tile_pass_tiles<V, EPI_HORNER, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+        }
+    } else {
+        V *kacc = b.acc + in.off;
+        a.c = (V)h;
+        a.acc_in = kacc;
+        a.acc_out = kacc;
+        if constexpr (P == 0) {
+            a.gx = x; a.out = u; a.cy = (V)0.5; a.stage = 0;
+            tile_pass_tiles<V, EPI_STAGE, true, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+        } else if constexpr (P == 1) {
+            a.gx = u; a.out = v; a.cy = (V)0.5; a.stage = 1;
+            tile_pass_tiles<V, EPI_STAGE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+        } else if constexpr (P == 2) {
+            a.gx = v; a.out = u; a.cy = (V)1.0; a.stage = 2;
+            tile_pass_tiles<V, EPI_STAGE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+        } else {
+            a.gx = u; a.out = x;
+            tile_pass_tiles<V, EPI_RK4ACC, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
         }
     }
 }

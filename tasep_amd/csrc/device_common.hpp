@@ -15,6 +15,13 @@ constexpr int kWaves = kBlock / kWave;
 constexpr int kMaxN = 34;
 constexpr int kRateStride = 40;  // doubles per rate set (n+1 <= 35)
 
+// Epilogues of the generator passes (kernels.hpp, tile_kernels.hpp):
+// EPI_STORE: out = self + c * (L G).  EPI_HORNER: x += c * (L G) with the
+// step's checksum and mass.  EPI_RK4: the staged step's combination from
+// stored k1, k2, k3.  EPI_STAGE / EPI_RK4ACC (tiled passes): the staged step
+// with materialised stage inputs and a running k-sum.
+enum { EPI_STORE = 0, EPI_HORNER = 1, EPI_RK4 = 2, EPI_STAGE = 3, EPI_RK4ACC = 4 };
+
 // Device-resident loop state.  Written by the end-of-step reduction (one
 // thread of k_finish_step / k_batch_finish, or of the LDS-resident kernel)
 // and by the host between runs; read by every kernel at entry.

@@ -188,12 +188,8 @@ __global__ __launch_bounds__(kFinishThreads) void k_finish_step(RunState *st, co
 // Rows, stage combination and update are computed in the storage type V
 // (FP32 contexts: FP32, as in the tiled, LDS-resident and batch kernels);
 // the checksum and mass are FP64 sums.
+// The epilogue (EPI_*) is a template parameter (device_common.hpp).
 // ===========================================================================
-// EPI_STORE: out = self + c * (L G).  EPI_HORNER: x += c * (L G) with the
-// step's checksum and mass.  EPI_RK4: the staged step's combination from
-// stored k1, k2, k3.  EPI_STAGE / EPI_RK4ACC (tiled passes): the staged step
-// with materialised stage inputs and a running k-sum (see tile_kernels.hpp).
-enum { EPI_STORE = 0, EPI_HORNER = 1, EPI_RK4 = 2, EPI_STAGE = 3, EPI_RK4ACC = 4 };
 
 template <class V>
 struct PassArgs {

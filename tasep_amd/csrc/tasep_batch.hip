@@ -50,6 +50,7 @@ struct tasep_batch {
     std::vector<int> str_ids;
     BatchInst *btab = nullptr;
     BatchBlock *blocks = nullptr;
+    BatchRange *ranges = nullptr;  // schedule 3: one tile range per workgroup
     uint32_t nblocks = 0;       // workgroups per pass (with padding)
     double *pchk = nullptr, *pmass = nullptr;
     hipStream_t stream = nullptr, stream2 = nullptr;
@@ -58,8 +59,10 @@ struct tasep_batch {
     // options
     int graph_steps = 32;       // steps per captured graph (0: direct launches)
     int use_lds = 1;
-    int schedule = 0;           // shared passes: 0 autotuned, 1 one lane per state, 2 tiles
+    int schedule = 0;           // shared passes: 0 autotuned, 1 one lane per state, 2 tiles, 3 big tiles
     int chosen = 1;             // schedule the current block table is built for
+    int range_tiles = 2;        // schedule 3: tiles per workgroup (config-4 batch, per step: 1 356 us, 2 331, 4 344, 8 353)
+    int store = 0;              // schedule 3 output stores: 0 by vector size, 1 non-temporal, 2 ordinary
     int concurrent = 1;
     hipGraphExec_t graph = nullptr;
     int graph_built_steps = 0;
@@ -97,6 +100,28 @@ bool tiles_fit(const tasep_batch *b) {
     return true;
 }
 
+// log2 of the big tile (schedule 3): the single-lattice tile pass's.
+int bigtile_bits(const tasep_batch *b) {
+    return b->prec == TASEP_F32 ? TileGeom<float>::b : TileGeom<double>::b;
+}
+
+// The big-tile schedule needs two of its tiles per streamed instance.
+bool bigtiles_fit(const tasep_batch *b) {
+    for (int i = 0; i < b->count; i++)
+        if (!lds_class(b, i) && b->n[i] < bigtile_bits(b) + 1) return false;
+    return true;
+}
+
+// Work units of one pass over instance i: blocks (schedules 1, 2) or tile
+// ranges (schedule 3).
+uint32_t units_of(const tasep_batch *b, int i, int sched) {
+    if (sched == 3) {
+        const uint64_t nt = (uint64_t)1 << (b->n[i] - bigtile_bits(b));
+        return (uint32_t)((nt + b->range_tiles - 1) / b->range_tiles);
+    }
+    return (uint32_t)(((uint64_t)1 << b->n[i]) / states_per_block(b, sched));
+}
+
 // Instance tables: LDS class and streamed class, XCD-aware block table for
 // schedule `sched`.
 int build_tables(tasep_batch *b, int sched) {
@@ -129,7 +154,6 @@ int build_tables(tasep_batch *b, int sched) {
     // runs on XCD w % 8, so instance j's blocks go to the XCD with the
     // fewest blocks so far (largest instances first) at positions
     // 8 * k + xcd: each instance's vectors stay in one XCD's L2.
-    const uint32_t per = states_per_block(b, sched);
     b->chosen = sched;
     std::vector<BatchInst> bt(b->str_ids.size());
     uint32_t slots = 0;
@@ -141,7 +165,7 @@ int build_tables(tasep_batch *b, int sched) {
         bt[j].rate_off = i * kRateStride;
         bt[j].tv_off = i * kRateStride;
         bt[j].part_off = slots;
-        bt[j].nblocks = (uint32_t)(((uint64_t)1 << b->n[i]) / per);
+        bt[j].nblocks = units_of(b, i, sched);
         bt[j].id = i;
         slots += bt[j].nblocks;
     }
@@ -173,18 +197,36 @@ int build_tables(tasep_batch *b, int sched) {
     for (size_t x = 0; x < 8; x++)
         for (size_t k = 0; k < lane[x].size(); k++) tab[8 * k + x] = lane[x][k];
     b->nblocks = (uint32_t)tab.size();
+    // schedule 3: unit k of instance j is the tile range [k K, min((k+1) K, tiles))
+    std::vector<BatchRange> rt;
+    if (sched == 3) {
+        rt.resize(tab.size(), BatchRange{kNoInst, 0, 0, 0});
+        for (size_t w = 0; w < tab.size(); w++) {
+            if (tab[w].inst == kNoInst) continue;
+            const BatchInst &bi = bt[tab[w].inst];
+            const uint32_t nt = 1u << (bi.n - bigtile_bits(b)), K = (uint32_t)b->range_tiles;
+            const uint32_t t0 = tab[w].blk * K;
+            rt[w] = BatchRange{tab[w].inst, t0, std::min(K, nt - t0), bi.part_off + tab[w].blk};
+        }
+    }
     hipFree(b->btab);
     hipFree(b->blocks);
+    hipFree(b->ranges);
     hipFree(b->pchk);
     hipFree(b->pmass);
     b->btab = nullptr;
     b->blocks = nullptr;
+    b->ranges = nullptr;
     b->pchk = b->pmass = nullptr;
     if (!bt.empty()) {
         HIP_OK(hipMalloc(&b->btab, bt.size() * sizeof(BatchInst)));
         HIP_OK(hipMemcpy(b->btab, bt.data(), bt.size() * sizeof(BatchInst), hipMemcpyHostToDevice));
         HIP_OK(hipMalloc(&b->blocks, tab.size() * sizeof(BatchBlock)));
         HIP_OK(hipMemcpy(b->blocks, tab.data(), tab.size() * sizeof(BatchBlock), hipMemcpyHostToDevice));
+        if (!rt.empty()) {
+            HIP_OK(hipMalloc(&b->ranges, rt.size() * sizeof(BatchRange)));
+            HIP_OK(hipMemcpy(b->ranges, rt.data(), rt.size() * sizeof(BatchRange), hipMemcpyHostToDevice));
+        }
         HIP_OK(hipMalloc(&b->pchk, slots * sizeof(double)));
         HIP_OK(hipMalloc(&b->pmass, slots * sizeof(double)));
     }
@@ -216,8 +258,49 @@ BatchArgs<V> batch_args(tasep_batch *b, int pass) {
     return a;
 }
 
+template <class V>
+BatchBigArgs<V> bigtile_args(tasep_batch *b) {
+    BatchArgs<V> c = batch_args<V>(b, 0);
+    BatchBigArgs<V> a;
+    a.inst = c.inst;
+    a.ranges = b->ranges;
+    a.x = c.x; a.u = c.u; a.v = c.v; a.acc = c.acc;
+    a.rates = c.rates;
+    a.tv_type = c.tv_type;
+    a.tv_amp = c.tv_amp; a.tv_period = c.tv_period; a.tv_offset = c.tv_offset; a.tv_c = c.tv_c;
+    a.st = c.st;
+    a.pchk = c.pchk;
+    a.pmass = c.pmass;
+    return a;
+}
+
+// Output stores of the big-tile passes, as a single lattice's tiled passes
+// choose them from one vector's size (here the batch's concatenated vector):
+// ordinary stores for 16-128 MiB, non-temporal otherwise.
+bool bigtile_nt(const tasep_batch *b) {
+    if (b->store) return b->store == 1;
+    const uint64_t bytes = b->total * b->esize;
+    return bytes < ((uint64_t)16 << 20) || bytes > ((uint64_t)128 << 20);
+}
+
+template <class V, bool NT>
+void launch_bigtile(tasep_batch *b, int p, hipStream_t s) {
+    const BatchBigArgs<V> a = bigtile_args<V>(b);
+    const dim3 g(b->nblocks), t(kTileThreads);
+    switch (p) {
+    case 0: hipLaunchKernelGGL((k_batch_bigtile<V, 0, NT>), g, t, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_batch_bigtile<V, 1, NT>), g, t, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_batch_bigtile<V, 2, NT>), g, t, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_batch_bigtile<V, 3, NT>), g, t, 0, s, a); break;
+    }
+}
+
 void launch_pass(tasep_batch *b, int p, hipStream_t s) {
-    if (b->chosen == 2) {
+    if (b->chosen == 3) {
+        const bool nt = bigtile_nt(b);
+        if (b->prec == TASEP_F32) nt ? launch_bigtile<float, true>(b, p, s) : launch_bigtile<float, false>(b, p, s);
+        else nt ? launch_bigtile<double, true>(b, p, s) : launch_bigtile<double, false>(b, p, s);
+    } else if (b->chosen == 2) {
         if (b->prec == TASEP_F32)
             hipLaunchKernelGGL(k_batch_tile<float>, dim3(b->nblocks), dim3(kBatchTileThreads), 0, s,
                                batch_args<float>(b, p));
@@ -236,14 +319,16 @@ void launch_pass(tasep_batch *b, int p, hipStream_t s) {
 
 // Autotuner of the shared passes: time a side-effect-free pass (pass 2 of
 // the step writes only the work vectors, which the next step's first pass
-// rewrites) under both schedules and keep the faster.  Needs valid run
-// states (h, stopped) on the device.
+// rewrites) under every schedule that fits and keep the fastest.  Needs
+// valid run states (h, stopped) on the device.
 int autotune(tasep_batch *b) {
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
-    float ms[3] = {0, 0, 0};
-    for (int sc = 1; sc <= 2; sc++) {
+    float ms[4] = {0, 0, 0, 0};
+    int best = 1;
+    for (int sc = 1; sc <= 3; sc++) {
+        if ((sc == 2 && !tiles_fit(b)) || (sc == 3 && !bigtiles_fit(b))) continue;
         int rc = build_tables(b, sc);
         if (rc) return rc;
         launch_pass(b, 1, b->stream);
@@ -252,10 +337,11 @@ int autotune(tasep_batch *b) {
         HIP_OK(hipEventRecord(e1, b->stream));
         HIP_OK(hipEventSynchronize(e1));
         HIP_OK(hipEventElapsedTime(&ms[sc], e0, e1));
+        if (ms[sc] < ms[best]) best = sc;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    return build_tables(b, ms[2] < ms[1] ? 2 : 1);
+    return build_tables(b, best);
 }
 
 // The streamed instances' passes of one step and their run-state update.
@@ -311,6 +397,7 @@ int tasep_batch_destroy(tasep_batch *b) {
     hipFree(b->lds_tab);
     hipFree(b->btab);
     hipFree(b->blocks);
+    hipFree(b->ranges);
     hipFree(b->pchk);
     hipFree(b->pmass);
     hipFree(b->mom_part);
@@ -491,6 +578,8 @@ int tasep_batch_get_option(tasep_batch *b, int key, int64_t *value) {
     case TASEP_BATCH_OPT_LDS: *value = b->use_lds; break;
     case TASEP_BATCH_OPT_CONCURRENT: *value = b->concurrent; break;
     case TASEP_BATCH_OPT_SCHEDULE: *value = b->tables_dirty ? b->schedule : b->chosen; break;
+    case TASEP_BATCH_OPT_RANGE_TILES: *value = b->range_tiles; break;
+    case TASEP_BATCH_OPT_STORE: *value = b->store; break;
     default: return fail(TASEP_ERR_ARG, "unknown batch option");
     }
     return TASEP_OK;
@@ -510,9 +599,19 @@ int tasep_batch_set_option(tasep_batch *b, int key, int64_t value) {
         break;
     case TASEP_BATCH_OPT_CONCURRENT: b->concurrent = value != 0; break;
     case TASEP_BATCH_OPT_SCHEDULE:
-        if (value < 0 || value > 2) return fail(TASEP_ERR_ARG, "schedule: 0 (autotuned), 1 or 2");
+        if (value < 0 || value > 3) return fail(TASEP_ERR_ARG, "schedule: 0 (autotuned), 1, 2 or 3");
         b->schedule = (int)value;
         b->tables_dirty = true;
+        break;
+    case TASEP_BATCH_OPT_RANGE_TILES:
+        if (value < 1 || value > 64) return fail(TASEP_ERR_ARG, "tiles per workgroup: 1..64");
+        b->range_tiles = (int)value;
+        b->tables_dirty = true;
+        break;
+    case TASEP_BATCH_OPT_STORE:
+        if (value < 0 || value > 2) return fail(TASEP_ERR_ARG, "store policy: 0, 1 or 2");
+        b->store = (int)value;
+        drop_graph(b);
         break;
     default: return fail(TASEP_ERR_ARG, "unknown batch option");
     }
@@ -548,7 +647,8 @@ int tasep_batch_run(tasep_batch *b, double t0, double h, uint64_t steps, double 
         // the streamed set decides whether tiles fit (build once to know it)
         int rc = build_tables(b, 1);
         if (rc) return rc;
-        if (!tiles_fit(b)) sched = 1;
+        if (sched == 3 && !bigtiles_fit(b)) sched = 2;
+        if (sched == 2 && !tiles_fit(b)) sched = 1;
         if (sched == 0 && !b->str_ids.empty()) rc = autotune(b);
         else rc = build_tables(b, sched == 0 ? 1 : sched);
         if (rc) return rc;

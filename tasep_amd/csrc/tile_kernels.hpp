@@ -214,17 +214,18 @@ __device__ __forceinline__ int pop_bond(uint64_t &m, int LB0) {
     return k;
 }
 
+// One workgroup's share of a tiled pass: `count` tiles T0, T0 + step, ...
+// of the lattice described by `a`, with the pass's rates already in LDS
+// (rr) and the tile and reduction buffers supplied by the kernel.  The
+// step's final passes leave the workgroup's (sum |dx|, sum x) in partial
+// slot `part`.
 template <class V, int EPI, bool SELF, bool NT>
-__global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(const TileArgs<V> a) {
+__device__ __forceinline__ void tile_pass_tiles(const TileArgs<V> &a, const V *__restrict__ rr, V *tile,
+                                                double *red, uint64_t T0, uint64_t step, uint64_t count,
+                                                uint64_t part) {
     using G = TileGeom<V>;
     constexpr int E = G::E, e = G::e, LB = G::LB, C = G::C, TB0 = G::TB0;
-    if (a.st->stopped) return;
-    __shared__ V tile[kTileThreads * C * E];
-    __shared__ V rr[kRateStride];
-    __shared__ double red[kTileWaves];
     const int n = a.n;
-    if (threadIdx.x <= n) rr[threadIdx.x] = (V)a.rates[a.rset * kRateStride + threadIdx.x];
-    __syncthreads();
     const int t = threadIdx.x;
     const int l = t & 63;                                          // lane in the chunk
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);          // chunk (uniform per wave)
@@ -264,17 +265,14 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
     const V d_c0 = lane_hi ? (V)0 : r_c0;                          // out-flow for columns 1, 3
     const bool use_self = (EPI != EPI_STORE) || (!SELF && a.use_self);
     Kahan chk, mass;
-    const TileWalk walk = tile_walk(a.ntiles, a.xcd_order, a.xcd_res);
-    const uint64_t stride = gridDim.x;
 
-    uint64_t it = blockIdx.x, kt = 0;
     constexpr int NS = TASEP_TILE_SLOTS;
     using Str = Cols<V, C, E>;
     Str cur, gn;                                                   // own vectors: this tile, the next
-    if (it < a.ntiles)  // prologue: own vector of the first tile
-        issue_cols<V, E, C, LB>(a, (walk.base << TB0) + wofs, lo, cur);
-    for (; it < a.ntiles; it += stride, kt++) {
-        const uint64_t T = walk.base + kt * walk.step;
+    if (count > 0)  // prologue: own vector of the first tile
+        issue_cols<V, E, C, LB>(a, (T0 << TB0) + wofs, lo, cur);
+    for (uint64_t kt = 0; kt < count; kt++) {
+        const uint64_t T = T0 + kt * step;
         const uint64_t wb = (T << TB0) + wofs;                     // wave base (uniform)
         const bool t_lo = T & 1, t_hi = (T >> (tb - 1)) & 1;
         // ---- issue every global read of this tile in the order it is used:
@@ -301,8 +299,8 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
         if constexpr (kAcc) {
             if (EPI == EPI_RK4ACC || a.stage > 0) load_cols_x<V, E, C, LB>(a.acc_in, wb, lo, ac);
         }
-        const bool more = it + stride < a.ntiles;
-        if (more) issue_cols<V, E, C, LB>(a, ((T + walk.step) << TB0) + wofs, lo, gn);
+        const bool more = kt + 1 < count;
+        if (more) issue_cols<V, E, C, LB>(a, ((T + step) << TB0) + wofs, lo, gn);
         // ---- own vector to LDS (waited for only here: the first tile's
         // streams are issued before it has arrived)
         const V(&g)[C][E] = cur.g;
@@ -472,8 +470,28 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
     if constexpr (EPI == EPI_HORNER || EPI == EPI_RK4ACC) {
         const double bc = block_sum<kTileThreads>(chk.value(), red);
         const double bm = block_sum<kTileThreads>(mass.value(), red);
-        store_partials(a.pchk, a.pmass, bc, bm);
+        if (threadIdx.x == 0) {
+            a.pchk[part] = bc;
+            a.pmass[part] = bm;
+        }
     }
+}
+
+template <class V, int EPI, bool SELF, bool NT>
+__global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(const TileArgs<V> a) {
+    using TG = TileGeom<V>;
+    if (a.st->stopped) return;
+    __shared__ V tile[kTileThreads * TG::C * TG::E];
+    __shared__ V rr[kRateStride];
+    __shared__ double red[kTileWaves];
+    if (threadIdx.x <= a.n) rr[threadIdx.x] = (V)a.rates[a.rset * kRateStride + threadIdx.x];
+    __syncthreads();
+    // this workgroup's tiles: blockIdx.x, blockIdx.x + gridDim.x, ... < ntiles,
+    // placed along the walk
+    const TileWalk walk = tile_walk(a.ntiles, a.xcd_order, a.xcd_res);
+    const uint64_t g = blockIdx.x, G = gridDim.x;
+    const uint64_t count = g < a.ntiles ? (a.ntiles - g + G - 1) / G : 0;
+    tile_pass_tiles<V, EPI, SELF, NT>(a, rr, tile, red, walk.base, walk.step, count, g);
 }
 
 // ===========================================================================

@@ -45,18 +45,19 @@ def _oracle(n, spec, steps, h=0.01):
 NS = [2, 5, 8, 9, 10, 12, 13, 14, 15, 16, 17]
 
 
-@pytest.mark.parametrize("schedule", [0, 1, 2])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
 def test_batch_matches_oracle_per_instance(prec, tol, schedule):
     """Every instance against its own oracle run, under the autotuned (0),
-    one-lane-per-state (1) and tile (2) schedules of the shared passes."""
+    one-lane-per-state (1), tile (2) and big-tile (3) schedules of the shared
+    passes."""
     steps = 40
     b, spec = _setup(NS + NS[::-1], 512, prec, schedule=schedule)
     with b:
         infos = b.run(0.0, 0.01, steps)
         xs = [b.get_x(i) for i in range(b.count)]
         chosen = b.get_option(T.BATCH_OPT_SCHEDULE)
-    assert chosen in (1, 2) and (schedule == 0 or chosen == schedule)
+    assert chosen in (1, 2, 3) and (schedule == 0 or chosen == schedule)
     for i, n in enumerate(b.ns):
         ox = _oracle(n, spec[i], steps)
         err = np.max(np.abs(xs[i] - ox))
@@ -67,7 +68,7 @@ def test_batch_matches_oracle_per_instance(prec, tol, schedule):
         assert abs(infos[i].mass - 1.0) <= (1e-12 if prec == "f64" else 1e-5)
 
 
-@pytest.mark.parametrize("schedule", [1, 2])
+@pytest.mark.parametrize("schedule", [1, 2, 3])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
 def test_batch_large_instances_match_oracle(prec, tol, schedule):
     """Instances larger than the benchmark batch's (n = 18, 20, 22: up to
@@ -116,7 +117,8 @@ def test_batch_graph_and_streams_do_not_change_results():
     runs = []
     for opts in (dict(graph_steps=0, concurrent=0, schedule=1), dict(graph_steps=8, concurrent=1, schedule=1),
                  dict(graph_steps=8, concurrent=0, schedule=1), dict(graph_steps=8, schedule=2),
-                 dict(graph_steps=0, schedule=2)):
+                 dict(graph_steps=0, schedule=2), dict(graph_steps=8, schedule=3),
+                 dict(graph_steps=0, schedule=3), dict(graph_steps=8, schedule=3, range_tiles=1, store=1)):
         b, _ = _setup(ns, 3, "f32", **opts)
         with b:
             b.run(0.0, 0.01, steps)
@@ -126,6 +128,9 @@ def test_batch_graph_and_streams_do_not_change_results():
             assert np.array_equal(a, c)
     for a, c in zip(runs[3], runs[4]):     # tiles: graph replay
         assert np.array_equal(a, c)
+    for r in runs[6:]:                     # big tiles: graph replay, tile ranges, store policy
+        for a, c in zip(runs[5], r):
+            assert np.array_equal(a, c)
 
 
 def test_batch_streamed_small_instances_match_lds():
@@ -214,7 +219,7 @@ def test_config4_workload_matches_oracle_per_instance():
         chosen = b.get_option(T.BATCH_OPT_SCHEDULE)
         xs = [b.get_x(i) for i in range(len(ns))]
     print(f"config-4 batch: schedule {chosen}")
-    assert chosen in (1, 2)
+    assert chosen in (1, 2, 3)
     worst = 0.0
     for i, (n, (kind, val)) in enumerate(zip(ns, spec)):
         x0 = oracle.uniform_x0(n)
@@ -230,3 +235,30 @@ def test_config4_workload_matches_oracle_per_instance():
         assert abs(infos[i].mass - 1.0) <= 1e-5, (i, infos[i].mass)
         assert abs(xs[i].sum() - 1.0) <= 1e-5
     print(f"config-4 batch: max over instances of max|x - x_oracle| = {worst:.3e}")
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_bigtile_batch_equals_single_context_tiles_bitwise(prec):
+    """The big-tile schedule runs the single-lattice tile pass per instance
+    range: every instance (constant rates in Horner form, time-variant ones
+    in staged form) equals a single context on forced tiles bit for bit."""
+    ns = [14, 18, 15, 16, 19, 14]
+    steps = 13
+    b, spec = _setup(ns, 31, prec, schedule=3, range_tiles=3)
+    with b:
+        b.run(0.0, 0.01, steps)
+        assert b.get_option(T.BATCH_OPT_SCHEDULE) == 3
+        xs = [b.get_x(i) for i in range(len(ns))]
+    for i, n in enumerate(ns):
+        with T.Engine(n, prec, "rk4", "mf") as e:
+            e.set_option(T.OPT_TILED, 2)
+            e.set_option(T.OPT_LDS, 0)
+            if spec[i][0] == "tv":
+                e.set_time_variant(spec[i][1])
+            else:
+                e.set_rates(spec[i][1])
+            e.set_x0_uniform()
+            e.run(0.0, 0.01, steps, patience=-1)
+            x = e.get_x()
+        print(f"instance {i} n={n} {spec[i][0]} {prec}: max|batch - context| = {np.max(np.abs(xs[i] - x)):.3e}")
+        assert np.array_equal(xs[i].view(np.uint64), x.view(np.uint64)), (i, n, spec[i][0])
