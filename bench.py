@@ -568,6 +568,8 @@ def main():
                     help="steps of the FP32-vs-FP64 error run (the workload's full t = 0..50); 0 skips it")
     ap.add_argument("--config", type=int, default=0,
                     help="run one of the five benchmark workloads (1..5) at full size instead")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="workload 4: skip the sequential per-instance comparison loop")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -581,7 +583,7 @@ def main():
         fn = {1: config_1, 2: config_2, 3: config_3, 4: config_4, 5: config_5}.get(args.config)
         if fn is None:
             sys.exit(f"workload {args.config} is not available")
-        res = fn(T, ev)
+        res = fn(T, ev, sequential=not args.no_sequential) if args.config == 4 else fn(T, ev)
         progress(f"configuration {args.config} done")
         print(json.dumps({"config": args.config, "result": res}), flush=True)
         return
