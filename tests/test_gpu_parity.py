@@ -1042,3 +1042,48 @@ def test_errors_are_reported():
     with T.Engine(3, "f64", "ref", "mf") as e:
         with pytest.raises(T.TasepError):
             e.window_marginals()          # needs n >= 4
+
+
+@pytest.mark.parametrize("n,opts", [(10, dict(lds=1)), (16, dict(lds=0, tiled=0)), (20, dict(lds=0, tiled=1)),
+                                    (16, dict(lds=0, tiled=2))])
+def test_ref_loop_zero_rates_bit_exact_vs_oracle(n, opts):
+    """Rates of 0 (no entry, no exit, blocked bonds): every active term adds
+    0 * x = +0, which must leave each row sum as the reference's, on the
+    LDS, one-lane and tiled kernels."""
+    rng = np.random.default_rng(80 + n)
+    for zero in ((0,), (n,), (3, n - 4), (0, n)):
+        r = rng.uniform(0.2, 1.8, n + 1)
+        r[list(zero)] = 0.0
+        x0 = _product_np(n, _site_densities(n, 81 + n))
+        ox, oinfo = oracle.run_ref(n, r, x0, 0.0, 0.01, 30, patience=-1, use_csr=False)
+        with T.Engine(n, "f64", "ref", "mf") as e:
+            for k, v in opts.items():
+                e.set_option(getattr(T, "OPT_" + k.upper()), v)
+            e.set_rates(r)
+            e.set_x(x0)
+            info = e.run(0.0, 0.01, 30, patience=-1)
+            x = e.get_x()
+        assert info.steps_done == 30
+        assert _bits_equal(x, ox), (zero, float(np.max(np.abs(x - ox))))
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5)])
+def test_rk4_zero_rates_vs_oracle(prec, tol):
+    """Classical RK4 with blocked bonds and no exit: the mass stays 1 and
+    the state follows the oracle, relative to its largest entry (tiles at
+    n=18 and one lane at n=15)."""
+    for n in (15, 18):
+        r = np.random.default_rng(90 + n).uniform(0.2, 1.8, n + 1)
+        r[[2, n - 5, n]] = 0.0
+        x0 = oracle.uniform_x0(n)
+        ox, _ = oracle.run_rk4(n, r, x0, 0.0, 0.01, 25)
+        with T.Engine(n, prec, "rk4", "mf") as e:
+            e.set_option(T.OPT_LDS, 0)
+            e.set_rates(r)
+            e.set_x0_uniform()
+            info = e.run(0.0, 0.01, 25, patience=-1)
+            x = e.get_x()
+        rel = float(np.max(np.abs(x - ox)) / np.max(ox))
+        print(f"n={n} {prec}: max|x - x_oracle| / max x = {rel:.2e}")
+        assert rel <= tol
+        assert abs(info.mass - 1.0) <= (1e-12 if prec == "f64" else 1e-5)
