@@ -253,6 +253,33 @@ def cpu_baseline(n, rates_fn, h, budget_s=20.0, keep_loop=False):
     return out
 
 
+def ref_loop_vs_cpu(T, n, rates, h, steps, x0_state=None, gen="mf"):
+    """The drop-in reference loop on the GPU against the reference's own CSR
+    loop on the host (CpuReferenceLoop), `steps` steps from the uniform
+    state or from the basis state x0_state: bit-exactness and the largest
+    differences."""
+    import oracle
+    loop = CpuReferenceLoop(n, rates, h)
+    if x0_state is not None:
+        loop.x[:] = 0.0
+        loop.x[x0_state] = 1.0
+    threads = int(_omp().omp_get_max_threads())
+    for _ in range(steps):
+        loop.step(threads)
+    with T.Engine(n, "f64", "ref", gen) as e:
+        e.set_rates(rates)
+        if x0_state is None:
+            e.set_x0_uniform()
+        else:
+            e.set_x0_state(x0_state)
+        e.run(0.0, h, steps, patience=-1)
+        xg = e.get_x()
+    return {"n": n, "steps": steps, "generator": gen,
+            "bit_exact": bool(np.array_equal(xg.view(np.uint64), loop.x.view(np.uint64))),
+            "max_abs_x_diff": float(np.max(np.abs(xg - loop.x))),
+            "max_density_diff": float(np.max(np.abs(oracle.site_moments(n, xg) - oracle.site_moments(n, loop.x))))}
+
+
 def fp64_gpu_vs_cpu(T, loop, rates, h, rk4_steps=2):
     """FP64 GPU against FP64 CPU on the same workload: the drop-in reference
     loop (matrix-free tiles) over the CPU reference loop's steps, compared
@@ -335,6 +362,8 @@ def config_1(T, ev):
                        "algo_bytes_per_state_update": 16}
     res["cpu_baseline"] = cpu_baseline(n, lambda m: np.array([0.5] + [1.0] * (m - 1) + [0.5]), h, budget_s=5.0)
     res["speedup_vs_cpu"] = mf["state_updates_per_s"] / res["cpu_baseline"]["value"]
+    # the whole run against the reference's own loop on the host
+    res["fp64_gpu_vs_cpu"] = {g: ref_loop_vs_cpu(T, n, r, h, 10000, x0_state=0, gen=g) for g in ("mf", "csr")}
     return res
 
 
@@ -366,6 +395,9 @@ def config_2(T, ev):
     res["cpu_baseline"] = cpu_baseline(n, lambda m: np.random.default_rng(20).uniform(0.1, 2.0, m + 1), h,
                                        budget_s=15.0)
     res["speedup_ref_mf_vs_cpu"] = res["ref_mf"]["state_updates_per_s"] / res["cpu_baseline"]["value"]
+    # the first 200 steps against the reference's own loop on the host
+    res["fp64_gpu_vs_cpu"] = {g: ref_loop_vs_cpu(T, n, r, h, 200, gen=g) for g in ("mf", "csr")}
+    progress("config 2 FP64 GPU vs CPU compared")
     return res
 
 
@@ -518,12 +550,51 @@ def config_4(T, ev, steps=2000, h=0.01, sequential=True):
                              "max_abs_diff_vs_batched_sampled": maxdiff}
         out["speedup_batched_vs_sequential"] = seq_ms / ms
         progress(f"config 4 sequential: {seq_ms:.1f} ms")
+    out["errors"] = batch_errors(T, ns, spec, h, steps, xb)
+    progress("config 4 errors against FP64 done")
     ach = out["batched"]["streamed_algorithmic_GBps"]
-    out["roofline"] = {"bound": "hbm", "kernel": "k_batch_tile / k_batch_pass (run average, streamed instances)",
+    out["roofline"] = {"bound": "hbm", "kernel": "k_batch_bigtile / k_batch_tile / k_batch_pass (run average, streamed instances)",
                        "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
     out["cpu_baseline"] = cpu_batch_baseline(ns, spec, h)
     out["speedup_batched_vs_cpu"] = out["batched"]["state_updates_per_s"] / out["cpu_baseline"]["value"]
     return out
+
+
+def batch_errors(T, ns, spec, h, steps, xb):
+    """FP32 batch against the same batch in FP64 (every instance's densities)
+    and the sampled FP32 states against the FP64 oracle's RK4 on the host."""
+    import oracle
+    with T.Batch(ns, "f64") as b:
+        for i, (kind, val) in enumerate(spec):
+            if kind == "tv":
+                b.set_time_variant(i, val)
+            else:
+                b.set_rates(i, val)
+        b.run(0.0, h, steps)
+        y64 = [b.moments(i)[0] for i in range(len(ns))]
+    with T.Batch(ns, "f32") as b:
+        for i, (kind, val) in enumerate(spec):
+            if kind == "tv":
+                b.set_time_variant(i, val)
+            else:
+                b.set_rates(i, val)
+        b.run(0.0, h, steps)
+        y32 = [b.moments(i)[0] for i in range(len(ns))]
+    dens = max(float(np.max(np.abs(a - c))) for a, c in zip(y32, y64))
+    worst = 0.0
+    picked = []
+    for i, x in sorted(xb.items()):
+        n = ns[i]
+        if n > 16:
+            continue
+        kind, val = spec[i]
+        x0 = oracle.uniform_x0(n)
+        ox, _ = (oracle.run_rk4(n, None, x0, 0.0, h, steps, tv=val) if kind == "tv"
+                 else oracle.run_rk4(n, val, x0, 0.0, h, steps))
+        worst = max(worst, float(np.max(np.abs(x - ox))))
+        picked.append(n)
+    return {"max_density_err_f32_vs_f64_batch_all_instances": dens,
+            "max_abs_x_err_f32_vs_fp64_oracle_sampled": worst, "oracle_sampled_n": picked}
 
 
 def cpu_batch_baseline(ns, spec, h, every=37, budget_s=1.5):
