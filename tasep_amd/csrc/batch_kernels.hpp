@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_pass(const BatchArgs<V>
     constexpr int E = BatchGeom<V>::E;
     constexpr uint32_t B = kBatchThreads * E;
     __shared__ V rs[kRateStride];
-    __shared__ double red[kBatchThreads / kWave];
+    __shared__ double red[2 * kBatchThreads / kWave];
     const BatchBlock bb = a.blocks[blockIdx.x];
     if (bb.inst == 0xffffffffu) return;
     const BatchInst in = a.inst[bb.inst];
@@ -181,8 +181,8 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_pass(const BatchArgs<V>
         }
     }
     if (p == 3) {
-        const double bc = block_sum<kBatchThreads>(ck.value(), red);
-        const double bm = block_sum<kBatchThreads>(km.value(), red);
+        double bc = ck.value(), bm = km.value();
+        block_sum2<kBatchThreads>(bc, bm, red);
         if (t == 0) {
             a.pchk[in.part_off + bb.blk] = bc;
             a.pmass[in.part_off + bb.blk] = bm;
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kBatchTileThreads, 8) void k_batch_tile(const Batch
     constexpr int NT = kBatchTileThreads;
     __shared__ V tile[TS];
     __shared__ V rs[kRateStride];
-    __shared__ double red[NT / kWave];
+    __shared__ double red[2 * NT / kWave];
     const BatchBlock bb = a.blocks[blockIdx.x];
     if (bb.inst == 0xffffffffu) return;
     const BatchInst in = a.inst[bb.inst];
@@ -392,8 +392,8 @@ __global__ __launch_bounds__(kBatchTileThreads, 8) void k_batch_tile(const Batch
         }
     }
     if (p == 3) {
-        const double bc = block_sum<NT>(ck.value(), red);
-        const double bm = block_sum<NT>(km.value(), red);
+        double bc = ck.value(), bm = km.value();
+        block_sum2<NT>(bc, bm, red);
         if (t == 0) {
             a.pchk[in.part_off + bb.blk] = bc;
             a.pmass[in.part_off + bb.blk] = bm;
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_batch_bigtil
     if (st->stopped) return;
     __shared__ V tile[kTileThreads * G::C * G::E];
     __shared__ V rr[kRateStride];
-    __shared__ double red[kTileWaves];
+    __shared__ double red[2 * kTileWaves];
     const int n = in.n, t = threadIdx.x;
     const double h = st->h;
     // rates of this pass: constant, or r(t), r(t+h/2), r(t+h) for stages 1, 2-3, 4

@@ -74,6 +74,30 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
     return r;
 }
 
+// block_sum of two values at once: one pair of barriers, each result
+// bit-identical to block_sum's.  `red` must hold 2*NT/64 doubles.
+template <int NT = kBlock>
+__device__ __forceinline__ void block_sum2(double &a, double &b, double *red) {
+    constexpr int W = NT / kWave;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) {
+        red[w] = a;
+        red[W + w] = b;
+    }
+    __syncthreads();
+    double ra = 0.0, rb = 0.0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        ra += red[i];
+        rb += red[W + i];
+    }
+    a = ra;
+    b = rb;
+}
+
 // Advance a run state by one finished step with its checksum sum|dx| and
 // mass sum x, exactly as the reference loop does
 // (src/masterEquation.c:644,670-679): t += h, then the sub-threshold counter
@@ -102,14 +126,15 @@ template <int NT = kBlock>
 __device__ __forceinline__ void reduce_partials(const double *__restrict__ part_chk,
                                                 const double *__restrict__ part_mass, uint32_t nslots,
                                                 double &chk, double &mass) {
-    __shared__ double s_red[NT / kWave];
+    __shared__ double s_red[2 * NT / kWave];
     Kahan kc, km;
     for (uint32_t b = threadIdx.x; b < nslots; b += NT) {
         kc.add(part_chk[b]);
         km.add(part_mass[b]);
     }
-    chk = block_sum<NT>(kc.value(), s_red);
-    mass = block_sum<NT>(km.value(), s_red);
+    chk = kc.value();
+    mass = km.value();
+    block_sum2<NT>(chk, mass, s_red);
 }
 
 // Block partials of a step's final kernel: thread 0 stores the block's

@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -518,10 +519,28 @@ static int resident_slots(const void *kernel) {
     return per_cu * cus;
 }
 
+// Compute units of the current device (cached per device).
+static int device_cus() {
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (!v && hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+        cache[dev].store(v, std::memory_order_relaxed);
+    return v > 0 ? v : 0;
+}
+
 // Grid and tile order of a tiled pass.  Grid: 16 workgroups per resident
 // slot, which balances the tail (measured at n=28: 0.92 vs 1.03 ms for one
 // resident wave), at most kMaxBlocks partials for the end-of-step
-// reduction, rounded down to a multiple of 8 XCDs.  The RK4 passes (`wide`)
+// reduction, rounded down to a multiple of 8 XCDs; and at most ntiles/8
+// workgroups but no fewer than one per CU, so that where tiles are few
+// (mid n) each workgroup still walks several and the next tile's reads
+// overlap the current tile's LDS phase (per step, by grid: reference loop
+// FP64 n=22 / 23 / 24 24.3 / 44.9 / 89.3 us against 30.2 / 59.5 / 116.0
+// with one tile per workgroup; RK4 FP64 n=22..24 and FP32 n=23..25 12-16%
+// faster; within 3% of the best grid for n=21..27, unchanged at n >= 28).
+// The RK4 passes (`wide`)
 // take the largest power of two from 8192 up to kMaxBlocks workgroups that
 // still gives each 8 tiles or more (per step, 8192 vs 4096 workgroups: FP64
 // n=28 6.84 vs 6.92 ms, n=30 32.7 vs 33.7 ms, FP32 n=30 15.5 vs 15.6 ms;
@@ -546,6 +565,11 @@ static TileLaunch tile_launch(const tasep_ctx *c, const void *kernel, int &slots
         if (ntiles >= w * 8) cap = w;
     }
     if (cap > (uint64_t)kMaxBlocks) cap = kMaxBlocks;
+    const int cus = device_cus();
+    uint64_t few = ntiles / 8;
+    if (few < (uint64_t)cus) few = (uint64_t)cus;
+    few &= ~(uint64_t)7;
+    if (few >= 8 && cap > few) cap = few;
     if (c->tile_blocks > 0) cap = (uint64_t)c->tile_blocks;
     TileLaunch L;
     L.grid = ntiles < cap ? ntiles : cap;

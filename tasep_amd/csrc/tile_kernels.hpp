@@ -468,8 +468,8 @@ __device__ __forceinline__ void tile_pass_tiles(const TileArgs<V> &a, const V *_
         if (more) cur = gn;
     }
     if constexpr (EPI == EPI_HORNER || EPI == EPI_RK4ACC) {
-        const double bc = block_sum<kTileThreads>(chk.value(), red);
-        const double bm = block_sum<kTileThreads>(mass.value(), red);
+        double bc = chk.value(), bm = mass.value();
+        block_sum2<kTileThreads>(bc, bm, red);
         if (threadIdx.x == 0) {
             a.pchk[part] = bc;
             a.pmass[part] = bm;
@@ -483,7 +483,7 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
     if (a.st->stopped) return;
     __shared__ V tile[kTileThreads * TG::C * TG::E];
     __shared__ V rr[kRateStride];
-    __shared__ double red[kTileWaves];
+    __shared__ double red[2 * kTileWaves];
     if (threadIdx.x <= a.n) rr[threadIdx.x] = (V)a.rates[a.rset * kRateStride + threadIdx.x];
     __syncthreads();
     // this workgroup's tiles: blockIdx.x, blockIdx.x + gridDim.x, ... < ntiles,
@@ -528,22 +528,34 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
     using G = TileGeom<double>;
     constexpr int E = G::E, LB = G::LB, C = G::C, TB0 = G::TB0;
     static_assert(E == 2 && LB == 7 && TB0 == 9, "bond order below assumes the FP64 tile");
-    if (a.st->stopped) return;
     __shared__ double tile[kTileThreads * C * E];
     __shared__ double rr[kRateStride];
-    __shared__ double red[kTileWaves];
+    __shared__ double red[2 * kTileWaves];
     const int n = a.n;
-    if (threadIdx.x <= n) rr[threadIdx.x] = a.rates[threadIdx.x];
-    __syncthreads();
     const int t = threadIdx.x;
     const int l = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const uint32_t lo = (uint32_t)l * E;
+    const uint64_t wofs = (uint64_t)w << (n - 3);
+    const TileWalk walk = tile_walk(a.ntiles, a.xcd_order, a.xcd_res);
+    auto issue = [&](uint64_t base, double (&dst)[C][E]) {
+#pragma unroll
+        for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), dst[c]);
+    };
+    // Prologue: the rates, then the first tile's own vector, are in flight
+    // before anything waits (the stopped flag is a scalar load), so that a
+    // workgroup with a single tile (mid n) pays one memory latency, not three.
+    const double rate = t <= n ? a.rates[t] : 0.0;
+    uint64_t it = blockIdx.x, kt = 0;
+    double g[C][E];
+    if (it < a.ntiles) issue((walk.base << TB0) + wofs, g);
+    if (a.st->stopped) return;
+    if (t <= n) rr[t] = rate;
+    __syncthreads();
     const double alpha = rr[0], beta = rr[n];
     auto rb = [&](int k) { return rr[n - 1 - k]; };               // rate of bond k
     const int tb = n - 3 - TB0;
     const uint64_t tmask = tb > 1 ? (((uint64_t)1 << (tb - 1)) - 1) : 0;
-    const uint64_t wofs = (uint64_t)w << (n - 3);
     const uint32_t lds0 = ((uint32_t)w * C * 64 + (uint32_t)l) * E;
     constexpr uint32_t LCOL = 64 * E;
     const bool w0 = w & 1, w1 = (w >> 1) & 1, w2 = (w >> 2) & 1;
@@ -577,20 +589,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
     const double w_b6 = lane_hi ? rb(6) : 0.0;                    // bond 6 in-flow, columns 0, 2
     const double d_b6 = lane_hi ? 0.0 : rb(6);                    // bond 6 out-flow, columns 1, 3
     Kahan chk, mass;
-    const TileWalk walk = tile_walk(a.ntiles, a.xcd_order, a.xcd_res);
     const uint64_t stride = gridDim.x;
     const double h = a.h;
 
     // One tile bond pre-issued (bond n-4 first): FP64 n=20..26 3-5% faster
     // per step than two, the same at n=28; three are 3% slower.
     constexpr int NS = 1;
-    auto issue = [&](uint64_t base, double (&dst)[C][E]) {
-#pragma unroll
-        for (int c = 0; c < C; c++) ld16(a.x + base + lo + ((uint32_t)c << LB), dst[c]);
-    };
-    uint64_t it = blockIdx.x, kt = 0;
-    double g[C][E];
-    if (it < a.ntiles) issue((walk.base << TB0) + wofs, g);
     for (; it < a.ntiles; it += stride, kt++) {
         const uint64_t T = walk.base + kt * walk.step;
         const uint64_t wb = (T << TB0) + wofs;
@@ -775,8 +779,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
                 for (int j = 0; j < E; j++) g[c][j] = gn[c][j];
         }
     }
-    const double bc = block_sum<kTileThreads>(chk.value(), red);
-    const double bm = block_sum<kTileThreads>(mass.value(), red);
+    double bc = chk.value(), bm = mass.value();
+    block_sum2<kTileThreads>(bc, bm, red);
     store_partials(a.pchk, a.pmass, bc, bm);
 }
 
