@@ -439,50 +439,68 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_batch_bigtil
     const BatchRange r = b.ranges[blockIdx.x];
     if (r.inst == 0xffffffffu) return;
     const BatchInst in = b.inst[r.inst];
-    RunState *st = b.st + in.id;
-    if (st->stopped) return;
     __shared__ V tile[kTileThreads * G::C * G::E];
     __shared__ V rr[kRateStride];
     __shared__ double red[2 * kTileWaves];
     const int n = in.n, t = threadIdx.x;
+    // Issue order once the instance is known: its rate parameters, the
+    // range's first tile (the gathered vector of pass P is x, u, v, u in
+    // both RK4 forms), then the run state (scalar loads), so that a
+    // workgroup waits on one memory latency before its LDS phase.
+    double q = 0.0, amp = 0.0, per = 0.0, ofs = 0.0, cc = 0.0;
+    int typ = 0;
+    if (t <= n) {
+        if (in.tv) {
+            const int o = in.tv_off + t;
+            typ = b.tv_type[o];
+            amp = b.tv_amp[o];
+            per = b.tv_period[o];
+            ofs = b.tv_offset[o];
+            cc = b.tv_c[o];
+        } else {
+            q = b.rates[in.rate_off + t];
+        }
+    }
+    V *x = b.x + in.off, *u = b.u + in.off, *v = b.v + in.off;
+    TileArgs<V> a = {};
+    a.gx = P == 0 ? x : (P == 2 ? v : u);
+    Cols<V, G::C, G::E> first;
+    issue_cols<V, G::E, G::C, G::LB>(a, ((uint64_t)r.t0 << G::TB0) + ((uint64_t)__builtin_amdgcn_readfirstlane(t >> 6) << (n - 3)),
+                                      (uint32_t)(t & 63) * G::E, first);
+    RunState *st = b.st + in.id;
+    if (st->stopped) return;
     const double h = st->h;
     // rates of this pass: constant, or r(t), r(t+h/2), r(t+h) for stages 1, 2-3, 4
     if (t <= n) {
-        double q;
         if (in.tv) {
-            const int o = in.tv_off + t;
             const double t0 = st->t;
             const double tq = P == 0 ? t0 : (P < 3 ? t0 + 0.5 * h : t0 + h);
-            q = tv_rate(b.tv_type[o], b.tv_amp[o], b.tv_period[o], b.tv_offset[o], b.tv_c[o], tq);
-        } else {
-            q = b.rates[in.rate_off + t];
+            q = tv_rate(typ, amp, per, ofs, cc, tq);
         }
         rr[t] = (V)q;
     }
     __syncthreads();
-    TileArgs<V> a = {};
     a.n = n;
     a.ntiles = (uint64_t)1 << (n - G::b);
     a.st = st;
     a.pchk = b.pchk;
     a.pmass = b.pmass;
     a.use_self = 1;
-    V *x = b.x + in.off, *u = b.u + in.off, *v = b.v + in.off;
     a.x = x;
     if (!in.tv) {
         // Horner: u = x + h/4 Lx, v = x + h/3 Lu, u = x + h/2 Lv, x += h Lu
         if constexpr (P == 0) {
             a.gx = x; a.out = u; a.c = (V)(0.25 * h);
-            tile_pass_tiles<V, EPI_STORE, true, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+            tile_pass_from<V, EPI_STORE, true, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
         } else if constexpr (P == 1) {
             a.gx = u; a.out = v; a.c = (V)(h / 3.0);
-            tile_pass_tiles<V, EPI_STORE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+            tile_pass_from<V, EPI_STORE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
         } else if constexpr (P == 2) {
             a.gx = v; a.out = u; a.c = (V)(0.5 * h);
-            tile_pass_tiles<V, EPI_STORE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+            tile_pass_from<V, EPI_STORE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
         } else {
             a.gx = u; a.out = x; a.c = (V)h;
-            tile_pass_tiles<V, EPI_HORNER, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+            tile_pass_from<V, EPI_HORNER, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
         }
     } else {
         V *kacc = b.acc + in.off;
@@ -491,16 +509,16 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_batch_bigtil
         a.acc_out = kacc;
         if constexpr (P == 0) {
             a.gx = x; a.out = u; a.cy = (V)0.5; a.stage = 0;
-            tile_pass_tiles<V, EPI_STAGE, true, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+            tile_pass_from<V, EPI_STAGE, true, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
         } else if constexpr (P == 1) {
             a.gx = u; a.out = v; a.cy = (V)0.5; a.stage = 1;
-            tile_pass_tiles<V, EPI_STAGE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+            tile_pass_from<V, EPI_STAGE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
         } else if constexpr (P == 2) {
             a.gx = v; a.out = u; a.cy = (V)1.0; a.stage = 2;
-            tile_pass_tiles<V, EPI_STAGE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+            tile_pass_from<V, EPI_STAGE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
         } else {
             a.gx = u; a.out = x;
-            tile_pass_tiles<V, EPI_RK4ACC, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part);
+            tile_pass_from<V, EPI_RK4ACC, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
         }
     }
 }

@@ -151,9 +151,14 @@ int build_tables(tasep_batch *b, int sched) {
         HIP_OK(hipMemcpy(b->lds_tab, lt.data(), lt.size() * sizeof(LdsInstance), hipMemcpyHostToDevice));
     }
     // streamed instances: partial slots, then the block table.  Workgroup w
-    // runs on XCD w % 8, so instance j's blocks go to the XCD with the
-    // fewest blocks so far (largest instances first) at positions
-    // 8 * k + xcd: each instance's vectors stay in one XCD's L2.
+    // runs on XCD w % 8, so instance j's blocks go to the XCD with the least
+    // work so far, at positions 8 * k + xcd: each instance's vectors stay in
+    // one XCD's L2.  Work is weighed by the bytes per state and step of the
+    // instance's RK4 form (stored stages 68, Horner 44 in FP32), and the
+    // time-variant instances come first (larger first within each kind), so
+    // every XCD ends its pass with the cheap blocks (config-4 batch: the
+    // XCDs' work within 0.4% of each other instead of 23% apart; 329 -> 322
+    // us per step).
     b->chosen = sched;
     std::vector<BatchInst> bt(b->str_ids.size());
     uint32_t slots = 0;
@@ -171,25 +176,37 @@ int build_tables(tasep_batch *b, int sched) {
     }
     std::vector<size_t> order(bt.size());
     for (size_t j = 0; j < order.size(); j++) order[j] = j;
-    std::stable_sort(order.begin(), order.end(), [&](size_t p, size_t q) { return bt[p].nblocks > bt[q].nblocks; });
+    std::stable_sort(order.begin(), order.end(), [&](size_t p, size_t q) {
+        if (bt[p].tv != bt[q].tv) return bt[p].tv > bt[q].tv;
+        return bt[p].nblocks > bt[q].nblocks;
+    });
+    auto unit_cost = [&](size_t j) -> uint64_t { return bt[j].tv ? 68 : 44; };
     // An instance larger than an eighth of all blocks is split into eight
     // contiguous runs, one per XCD (a lone large lattice still fills the chip).
     uint64_t total_blocks = 0;
     for (const BatchInst &bi : bt) total_blocks += bi.nblocks;
     std::vector<std::vector<BatchBlock>> lane(8);
+    uint64_t work[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (size_t j : order) {
         const uint32_t nb = bt[j].nblocks;
+        const uint64_t uc = unit_cost(j);
         if (nb >= 8 && (uint64_t)nb * 8 > total_blocks) {
             for (uint32_t x = 0; x < 8; x++)
-                for (uint32_t k = x * (nb / 8); k < (x + 1) * (nb / 8); k++)
+                for (uint32_t k = x * (nb / 8); k < (x + 1) * (nb / 8); k++) {
                     lane[x].push_back(BatchBlock{(uint32_t)j, k});
-            for (uint32_t k = 8 * (nb / 8); k < nb; k++) lane[k % 8].push_back(BatchBlock{(uint32_t)j, k});
+                    work[x] += uc;
+                }
+            for (uint32_t k = 8 * (nb / 8); k < nb; k++) {
+                lane[k % 8].push_back(BatchBlock{(uint32_t)j, k});
+                work[k % 8] += uc;
+            }
             continue;
         }
         size_t best = 0;
         for (size_t x = 1; x < 8; x++)
-            if (lane[x].size() < lane[best].size()) best = x;
+            if (work[x] < work[best]) best = x;
         for (uint32_t k = 0; k < nb; k++) lane[best].push_back(BatchBlock{(uint32_t)j, k});
+        work[best] += uc * nb;
     }
     size_t longest = 0;
     for (auto &l : lane) longest = std::max(longest, l.size());

@@ -218,11 +218,12 @@ __device__ __forceinline__ int pop_bond(uint64_t &m, int LB0) {
 // of the lattice described by `a`, with the pass's rates already in LDS
 // (rr) and the tile and reduction buffers supplied by the kernel.  The
 // step's final passes leave the workgroup's (sum |dx|, sum x) in partial
-// slot `part`.
+// slot `part`.  `cur` holds the first tile's own vector, already issued by
+// the caller (tile_pass_tiles below issues it).
 template <class V, int EPI, bool SELF, bool NT>
-__device__ __forceinline__ void tile_pass_tiles(const TileArgs<V> &a, const V *__restrict__ rr, V *tile,
-                                                double *red, uint64_t T0, uint64_t step, uint64_t count,
-                                                uint64_t part) {
+__device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__restrict__ rr, V *tile,
+                                               double *red, uint64_t T0, uint64_t step, uint64_t count,
+                                               uint64_t part, Cols<V, TileGeom<V>::C, TileGeom<V>::E> cur) {
     using G = TileGeom<V>;
     constexpr int E = G::E, e = G::e, LB = G::LB, C = G::C, TB0 = G::TB0;
     const int n = a.n;
@@ -268,9 +269,7 @@ __device__ __forceinline__ void tile_pass_tiles(const TileArgs<V> &a, const V *_
 
     constexpr int NS = TASEP_TILE_SLOTS;
     using Str = Cols<V, C, E>;
-    Str cur, gn;                                                   // own vectors: this tile, the next
-    if (count > 0)  // prologue: own vector of the first tile
-        issue_cols<V, E, C, LB>(a, (T0 << TB0) + wofs, lo, cur);
+    Str gn;                                                        // own vector of the next tile
     for (uint64_t kt = 0; kt < count; kt++) {
         const uint64_t T = T0 + kt * step;
         const uint64_t wb = (T << TB0) + wofs;                     // wave base (uniform)
@@ -477,21 +476,44 @@ __device__ __forceinline__ void tile_pass_tiles(const TileArgs<V> &a, const V *_
     }
 }
 
+// The same, issuing the first tile's own vector itself.
+template <class V, int EPI, bool SELF, bool NT>
+__device__ __forceinline__ void tile_pass_tiles(const TileArgs<V> &a, const V *__restrict__ rr, V *tile,
+                                                double *red, uint64_t T0, uint64_t step, uint64_t count,
+                                                uint64_t part) {
+    using G = TileGeom<V>;
+    Cols<V, G::C, G::E> cur;
+    if (count > 0) {
+        const uint64_t wofs = (uint64_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) << (a.n - 3);
+        issue_cols<V, G::E, G::C, G::LB>(a, (T0 << G::TB0) + wofs, (uint32_t)(threadIdx.x & 63) * G::E, cur);
+    }
+    tile_pass_from<V, EPI, SELF, NT>(a, rr, tile, red, T0, step, count, part, cur);
+}
+
 template <class V, int EPI, bool SELF, bool NT>
 __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(const TileArgs<V> a) {
     using TG = TileGeom<V>;
-    if (a.st->stopped) return;
     __shared__ V tile[kTileThreads * TG::C * TG::E];
     __shared__ V rr[kRateStride];
     __shared__ double red[2 * kTileWaves];
-    if (threadIdx.x <= a.n) rr[threadIdx.x] = (V)a.rates[a.rset * kRateStride + threadIdx.x];
-    __syncthreads();
     // this workgroup's tiles: blockIdx.x, blockIdx.x + gridDim.x, ... < ntiles,
     // placed along the walk
     const TileWalk walk = tile_walk(a.ntiles, a.xcd_order, a.xcd_res);
     const uint64_t g = blockIdx.x, G = gridDim.x;
     const uint64_t count = g < a.ntiles ? (a.ntiles - g + G - 1) / G : 0;
-    tile_pass_tiles<V, EPI, SELF, NT>(a, rr, tile, red, walk.base, walk.step, count, g);
+    // the rates, then the first tile's own vector, are in flight before
+    // anything waits (the stopped flag is a scalar load)
+    const int t = threadIdx.x;
+    const double rate = t <= a.n ? a.rates[a.rset * kRateStride + t] : 0.0;
+    Cols<V, TG::C, TG::E> first;
+    if (count > 0) {
+        const uint64_t wofs = (uint64_t)__builtin_amdgcn_readfirstlane(t >> 6) << (a.n - 3);
+        issue_cols<V, TG::E, TG::C, TG::LB>(a, (walk.base << TG::TB0) + wofs, (uint32_t)(t & 63) * TG::E, first);
+    }
+    if (a.st->stopped) return;
+    if (t <= a.n) rr[t] = (V)rate;
+    __syncthreads();
+    tile_pass_from<V, EPI, SELF, NT>(a, rr, tile, red, walk.base, walk.step, count, g, first);
 }
 
 // ===========================================================================

@@ -744,8 +744,9 @@ def test_tile_orders_bitwise_equal(case):
     """Every tile order (grid, XCD eighths, per-XCD sweep, device sweep),
     grid size and output store policy gives the same bits: each state's row
     is summed in a fixed order whichever workgroup owns its tile.  n=22 with
-    256 workgroups gives every workgroup several tiles, so the sweeps are
-    really taken."""
+    256 workgroups (also the default grid there) gives every workgroup
+    several tiles, so the sweeps are really taken; 1024 gives each FP64
+    workgroup one tile (FP32: 512 workgroups, one tile each)."""
     n = 22
     prec, kind = case.split("-", 1)
     rng = np.random.default_rng(23)
@@ -754,7 +755,7 @@ def test_tile_orders_bitwise_equal(case):
               period=rng.uniform(0.5, 2, n + 1), offset=rng.uniform(0, 1, n + 1), c=np.ones(n + 1))
     out = []
     for order, blocks, store in ((1, 0, 0), (0, 256, 0), (1, 256, 0), (2, 256, 0), (3, 256, 0), (2, 0, 0),
-                                 (3, 0, 0), (2, 0, 1), (2, 0, 2)):
+                                 (3, 0, 0), (2, 0, 1), (2, 0, 2), (2, 1024, 0), (1, 1024, 0)):
         with T.Engine(n, prec, "ref" if kind == "ref" else "rk4", "mf") as e:
             e.set_option(T.OPT_XCD_ORDER, order)
             e.set_option(T.OPT_TILE_BLOCKS, blocks)
