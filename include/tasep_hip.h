@@ -195,10 +195,16 @@ int tasep_step_profile(tasep_ctx *ctx, double t0, double h, int reps,
  *                              (RK4) / n >= 19 (reference loop), 2 wherever
  *                              a lattice has two tiles (n >= 14 FP32,
  *                              n >= 13 FP64), 0 = one lane per state
- *   TASEP_OPT_XCD_ORDER    [2] order of the tiles in the tiled passes: 0 grid
+ *   TASEP_OPT_XCD_ORDER    [4] order of the tiles in the tiled passes: 0 grid
  *                              order, 1 contiguous eighths per XCD, 2 eighths
  *                              per XCD each swept in order by the workgroups
- *                              resident on it, 3 one sweep over all tiles
+ *                              resident on it, 3 one sweep over all tiles,
+ *                              4 as 2 over a level walk: tiles grouped by the
+ *                              particles on their upper tile-index bits (by
+ *                              count, then position sum), so that a tile and
+ *                              the tiles it reads across tile bonds are
+ *                              walked close together (a table of 4 bytes per
+ *                              tile, built on first use)
  *   TASEP_OPT_RK4_STAGED   [0] RK4 with four stored stages even for constant
  *                              rates (default: the Horner form of the same
  *                              step, 3 work vectors instead of 5)

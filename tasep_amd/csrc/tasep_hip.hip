@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <mutex>
 #include <vector>
@@ -49,7 +50,9 @@ struct tasep_ctx {
     // options (tasep_set_option)
     int tiled = 1;              // tiled matrix-free passes: 0 never, 1 n >= 18 (19 REF), 2 where they fit
     bool graphs = false;        // long runs replay a captured graph of steps
-    int xcd_order = 2;          // tile order of the tiled passes (tile_walk)
+    int xcd_order = 4;          // tile order of the tiled passes (tile_walk; 4: level walk)
+    uint32_t *walk = nullptr;   // level-walk table of the context's tiles (built on first use)
+    int walk_tb = -1;           // tile-index bits the table was built for
     bool force_staged = false;  // staged RK4 even for constant rates
     int tile_blocks = 0;        // persistent grid of the tiled passes (0: by occupancy)
     int lds = 1;                // whole runs in one workgroup's LDS: 0 never, 1 n <= 12, 2 where it fits
@@ -117,6 +120,7 @@ int tasep_ctx_destroy(tasep_ctx *c) {
     hipFree(c->mom_part);
     hipFree(c->mom_out);
     hipFree(c->lds_inst);
+    hipFree(c->walk);
     if (c->lds_inst_host) hipHostFree(c->lds_inst_host);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -550,10 +554,63 @@ static int device_cus() {
 // grid allows it, else the next simpler one.
 struct TileLaunch {
     uint64_t grid;
-    int order;
+    int order;                  // order the kernel walks its positions in (0..3)
     uint32_t res;
+    const uint32_t *walk;       // level-walk table over the positions of order 2, or null
 };
-static TileLaunch tile_launch(const tasep_ctx *c, const void *kernel, int &slots_cache, uint64_t ntiles,
+
+// Level-walk table for tiles with tb index bits: walk[p] is the tile at
+// position p.  Bit 0 of the tile index varies fastest; the bits above it
+// are read as particles on tb - 1 sites and ordered by their count, then by
+// the sum of their positions, ascending within odd sums and descending
+// within even ones.  A hop across a bond between two of those bits keeps
+// the count and raises the position sum by one, so the tile read across
+// such a bond (the configuration before the hop) comes in the next group
+// of the walk.  In an LRU model of one XCD's L2 replaying a pass's reads
+// the gathered vector is fetched 2.78 instead of 3.54 times per pass at
+// n = 28 (FP32 tiles), 3.08 instead of 3.81 (FP64), 3.95 instead of 4.62
+// at n = 32.
+static void level_walk(int tb, std::vector<uint32_t> &walk) {
+    const int F = tb - 1;
+    const uint32_t np = (uint32_t)1 << F;
+    const int nb = (F + 1) * (F * (F - 1) / 2 + 1);               // (count, position sum) groups
+    auto group = [F](uint32_t p) {
+        int m = 0, sum = 0;
+        for (int i = 0; i < F; i++)
+            if ((p >> i) & 1) { m++; sum += i; }
+        return m * (F * (F - 1) / 2 + 1) + sum;
+    };
+    std::vector<uint32_t> start((size_t)nb + 1, 0);
+    for (uint32_t p = 0; p < np; p++) start[group(p) + 1]++;
+    for (int b = 0; b < nb; b++) start[b + 1] += start[b];
+    std::vector<uint32_t> fill(start.begin(), start.end() - 1), pats(np);
+    for (uint32_t p = 0; p < np; p++) pats[fill[group(p)]++] = p;  // ascending within a group
+    const int S = F * (F - 1) / 2 + 1;
+    for (int b = 0; b < nb; b++)
+        if ((b % S) % 2 == 0) std::reverse(pats.begin() + start[b], pats.begin() + start[b + 1]);
+    walk.resize((size_t)np * 2);
+    for (uint32_t i = 0; i < np; i++) {
+        walk[2 * i] = pats[i] << 1;
+        walk[2 * i + 1] = (pats[i] << 1) | 1;
+    }
+}
+
+static const uint32_t *ensure_walk(tasep_ctx *c, int tb) {
+    if (c->walk_tb == tb) return c->walk;
+    std::vector<uint32_t> w;
+    level_walk(tb, w);
+    uint32_t *d = nullptr;
+    if (hipMalloc(&d, w.size() * sizeof(uint32_t)) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, w.data(), w.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        return nullptr;
+    }
+    hipFree(c->walk);
+    c->walk = d;
+    c->walk_tb = tb;
+    return d;
+}
+static TileLaunch tile_launch(tasep_ctx *c, const void *kernel, int &slots_cache, uint64_t ntiles,
                               bool wide = false) {
     if (!slots_cache) slots_cache = resident_slots(kernel);
     const int slots = slots_cache;
@@ -574,6 +631,19 @@ static TileLaunch tile_launch(const tasep_ctx *c, const void *kernel, int &slots
     TileLaunch L;
     L.grid = ntiles < cap ? ntiles : cap;
     L.order = c->xcd_order;
+    L.walk = nullptr;
+    if (L.order == 4) {
+        // the table permutes the positions of order 2
+        L.res = slots >= 8 ? (uint32_t)(slots / 8) : 1;
+        if ((L.grid % 8) == 0 && ((L.grid / 8) % L.res) == 0 && (ntiles % L.grid) == 0 && ntiles >= 2) {
+            L.walk = ensure_walk(c, 63 - __builtin_clzll(ntiles));
+            if (L.walk) {
+                L.order = 2;
+                return L;
+            }
+        }
+        L.order = 2;
+    }
     if (L.order == 3) {
         L.res = slots > 0 ? (uint32_t)slots : 1;
         if ((L.grid % L.res) == 0 && (ntiles % L.grid) == 0) return L;
@@ -591,6 +661,7 @@ static uint32_t launch_tile_nt(tasep_ctx *c, TileArgs<V> a) {
     const TileLaunch L = tile_launch(c, (const void *)k_tile_pass<V, EPI, SELF, NT>, slots, a.ntiles, true);
     a.xcd_order = L.order;
     a.xcd_res = L.res;
+    a.walk = L.walk;
     hipLaunchKernelGGL((k_tile_pass<V, EPI, SELF, NT>), dim3(L.grid), dim3(kTileThreads), 0, c->stream, a);
     return (uint32_t)L.grid;
 }
@@ -685,6 +756,7 @@ static void ref_step(tasep_ctx *c, double h, uint64_t j) {
         const uint64_t g = L.grid;
         t.xcd_order = L.order;
         t.xcd_res = L.res;
+        t.walk = L.walk;
         prof_begin(c);
         if (nt) hipLaunchKernelGGL(k_tile_ref<true>, dim3(g), dim3(kTileThreads), 0, c->stream, t);
         else hipLaunchKernelGGL(k_tile_ref<false>, dim3(g), dim3(kTileThreads), 0, c->stream, t);
@@ -960,7 +1032,7 @@ int tasep_set_option(tasep_ctx *c, int key, int64_t value) {
         c->tiled = (int)value;
         break;
     case TASEP_OPT_XCD_ORDER:
-        if (value < 0 || value > 3) return fail(TASEP_ERR_ARG, "tile order: 0 to 3");
+        if (value < 0 || value > 4) return fail(TASEP_ERR_ARG, "tile order: 0 to 4");
         c->xcd_order = (int)value;
         break;
     case TASEP_OPT_RK4_STAGED: c->force_staged = value != 0; break;

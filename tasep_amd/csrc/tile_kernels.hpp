@@ -60,6 +60,10 @@ constexpr int kTileWaves = kTileThreads / kWave;
 #ifndef TASEP_TILE_SLOTS
 #define TASEP_TILE_SLOTS 2
 #endif
+// The same for the reference-loop tiles.
+#ifndef TASEP_REF_SLOTS
+#define TASEP_REF_SLOTS 1
+#endif
 
 template <class V> struct TileGeom;
 // E states per lane per column (one 16-byte load), C = 4 columns per lane.
@@ -145,6 +149,7 @@ struct TileArgs {
     uint64_t ntiles;
     int xcd_order;     // tile order (tile_walk)
     uint32_t xcd_res;  // workgroups resident per XCD (order 2)
+    const uint32_t *walk;  // tile at each walk position (level walk), or null: the position itself
 };
 
 
@@ -175,7 +180,8 @@ __device__ __forceinline__ void load_cols_x(const V *p, uint64_t base, uint32_t 
     for (int c = 0; c < C; c++) ld_self<V, E>(p + base + lo + ((uint32_t)c << LB), g[c]);
 }
 
-// Tiles of one workgroup: its k-th tile is base + k * step.
+// Walk positions of one workgroup: its k-th position is base + k * step;
+// the tile at position p is walk[p] with a level-walk table, else p.
 //   order 0: grid order, T = g + k * G
 //   order 1: XCD eighths -- workgroup g runs on XCD g % 8 and walks that
 //            XCD's eighth of the tile range with stride G / 8
@@ -189,9 +195,17 @@ __device__ __forceinline__ void load_cols_x(const V *p, uint64_t base, uint32_t 
 //            2^j tiles ahead is read again 2^j tiles of traffic later (a
 //            window the memory-side cache holds for larger j than the
 //            per-XCD sweeps).  Needs G % R == 0 and ntiles % G == 0.
+//   The level walk (a table over the positions of order 2) groups tiles
+//   by the particles on their tile bits above the lowest: by count, then by
+//   position sum, alternately ascending and descending within a sum.  A
+//   hop between two such bits keeps the count and raises the sum by one,
+//   so a tile-bond partner sits in the next group, not 2^j tiles ahead.
 struct TileWalk {
     uint64_t base, step;
 };
+__device__ __forceinline__ uint64_t tile_at(const uint32_t *walk, uint64_t p) {
+    return walk ? (uint64_t)walk[p] : p;
+}
 __device__ __forceinline__ TileWalk tile_walk(uint64_t ntiles, int order, uint32_t res) {
     const uint64_t g = blockIdx.x, G = gridDim.x;
     if (order == 0) return {g, G};
@@ -214,15 +228,15 @@ __device__ __forceinline__ int pop_bond(uint64_t &m, int LB0) {
     return k;
 }
 
-// One workgroup's share of a tiled pass: `count` tiles T0, T0 + step, ...
-// of the lattice described by `a`, with the pass's rates already in LDS
+// One workgroup's share of a tiled pass: the tiles at walk positions P0,
+// P0 + step, ... (`count` of them) of the lattice described by `a`, with the pass's rates already in LDS
 // (rr) and the tile and reduction buffers supplied by the kernel.  The
 // step's final passes leave the workgroup's (sum |dx|, sum x) in partial
 // slot `part`.  `cur` holds the first tile's own vector, already issued by
 // the caller (tile_pass_tiles below issues it).
 template <class V, int EPI, bool SELF, bool NT>
 __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__restrict__ rr, V *tile,
-                                               double *red, uint64_t T0, uint64_t step, uint64_t count,
+                                               double *red, uint64_t P0, uint64_t step, uint64_t count,
                                                uint64_t part, Cols<V, TileGeom<V>::C, TileGeom<V>::E> cur) {
     using G = TileGeom<V>;
     constexpr int E = G::E, e = G::e, LB = G::LB, C = G::C, TB0 = G::TB0;
@@ -270,8 +284,11 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
     constexpr int NS = TASEP_TILE_SLOTS;
     using Str = Cols<V, C, E>;
     Str gn;                                                        // own vector of the next tile
+    uint64_t T = tile_at(a.walk, P0);
+    uint64_t Tn = count > 1 ? tile_at(a.walk, P0 + step) : 0;
     for (uint64_t kt = 0; kt < count; kt++) {
-        const uint64_t T = T0 + kt * step;
+        // the tile after next, read a whole tile ahead of its use
+        const uint64_t Tnn = kt + 2 < count ? tile_at(a.walk, P0 + (kt + 2) * step) : 0;
         const uint64_t wb = (T << TB0) + wofs;                     // wave base (uniform)
         const bool t_lo = T & 1, t_hi = (T >> (tb - 1)) & 1;
         // ---- issue every global read of this tile in the order it is used:
@@ -299,7 +316,7 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
             if (EPI == EPI_RK4ACC || a.stage > 0) load_cols_x<V, E, C, LB>(a.acc_in, wb, lo, ac);
         }
         const bool more = kt + 1 < count;
-        if (more) issue_cols<V, E, C, LB>(a, ((T + step) << TB0) + wofs, lo, gn);
+        if (more) issue_cols<V, E, C, LB>(a, (Tn << TB0) + wofs, lo, gn);
         // ---- own vector to LDS (waited for only here: the first tile's
         // streams are issued before it has arrived)
         const V(&g)[C][E] = cur.g;
@@ -465,6 +482,8 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
             st_out<NT>(a.out + wb + lo + ((uint32_t)c << LB), o);
         }
         if (more) cur = gn;
+        T = Tn;
+        Tn = Tnn;
     }
     if constexpr (EPI == EPI_HORNER || EPI == EPI_RK4ACC) {
         double bc = chk.value(), bm = mass.value();
@@ -479,15 +498,16 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
 // The same, issuing the first tile's own vector itself.
 template <class V, int EPI, bool SELF, bool NT>
 __device__ __forceinline__ void tile_pass_tiles(const TileArgs<V> &a, const V *__restrict__ rr, V *tile,
-                                                double *red, uint64_t T0, uint64_t step, uint64_t count,
+                                                double *red, uint64_t P0, uint64_t step, uint64_t count,
                                                 uint64_t part) {
     using G = TileGeom<V>;
     Cols<V, G::C, G::E> cur;
     if (count > 0) {
         const uint64_t wofs = (uint64_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) << (a.n - 3);
-        issue_cols<V, G::E, G::C, G::LB>(a, (T0 << G::TB0) + wofs, (uint32_t)(threadIdx.x & 63) * G::E, cur);
+        issue_cols<V, G::E, G::C, G::LB>(a, (tile_at(a.walk, P0) << G::TB0) + wofs,
+                                         (uint32_t)(threadIdx.x & 63) * G::E, cur);
     }
-    tile_pass_from<V, EPI, SELF, NT>(a, rr, tile, red, T0, step, count, part, cur);
+    tile_pass_from<V, EPI, SELF, NT>(a, rr, tile, red, P0, step, count, part, cur);
 }
 
 template <class V, int EPI, bool SELF, bool NT>
@@ -508,7 +528,8 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
     Cols<V, TG::C, TG::E> first;
     if (count > 0) {
         const uint64_t wofs = (uint64_t)__builtin_amdgcn_readfirstlane(t >> 6) << (a.n - 3);
-        issue_cols<V, TG::E, TG::C, TG::LB>(a, (walk.base << TG::TB0) + wofs, (uint32_t)(t & 63) * TG::E, first);
+        issue_cols<V, TG::E, TG::C, TG::LB>(a, (tile_at(a.walk, walk.base) << TG::TB0) + wofs,
+                                            (uint32_t)(t & 63) * TG::E, first);
     }
     if (a.st->stopped) return;
     if (t <= a.n) rr[t] = (V)rate;
@@ -543,6 +564,7 @@ struct TileRefArgs {
     uint64_t ntiles;
     int xcd_order;
     uint32_t xcd_res;
+    const uint32_t *walk;  // tile at each walk position (level walk), or null
 };
 
 template <bool NT>
@@ -570,7 +592,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
     const double rate = t <= n ? a.rates[t] : 0.0;
     uint64_t it = blockIdx.x, kt = 0;
     double g[C][E];
-    if (it < a.ntiles) issue((walk.base << TB0) + wofs, g);
+    uint64_t T = it < a.ntiles ? tile_at(a.walk, walk.base) : 0;
+    if (it < a.ntiles) issue((T << TB0) + wofs, g);
     if (a.st->stopped) return;
     if (t <= n) rr[t] = rate;
     __syncthreads();
@@ -616,9 +639,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
 
     // One tile bond pre-issued (bond n-4 first): FP64 n=20..26 3-5% faster
     // per step than two, the same at n=28; three are 3% slower.
-    constexpr int NS = 1;
+    constexpr int NS = TASEP_REF_SLOTS;
+    uint64_t Tn = it + stride < a.ntiles ? tile_at(a.walk, walk.base + walk.step) : 0;
     for (; it < a.ntiles; it += stride, kt++) {
-        const uint64_t T = walk.base + kt * walk.step;
+        // the tile after next, read a whole tile ahead of its use
+        const uint64_t Tnn = it + 2 * stride < a.ntiles ? tile_at(a.walk, walk.base + (kt + 2) * walk.step) : 0;
         const uint64_t wb = (T << TB0) + wofs;
         const bool t_lo = T & 1, t_hi = (T >> (tb - 1)) & 1;
         // ---- issue every global read of this tile before its LDS phase:
@@ -642,7 +667,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
         }
         double gn[C][E];
         const bool more = it + stride < a.ntiles;
-        if (more) issue(((T + walk.step) << TB0) + wofs, gn);
+        if (more) issue((Tn << TB0) + wofs, gn);
         __syncthreads();  // the previous tile's LDS reads are done
 #pragma unroll
         for (int c = 0; c < C; c++) st16(&tile[lds0 + c * LCOL], g[c]);
@@ -800,6 +825,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
 #pragma unroll
                 for (int j = 0; j < E; j++) g[c][j] = gn[c][j];
         }
+        T = Tn;
+        Tn = Tnn;
     }
     double bc = chk.value(), bm = mass.value();
     block_sum2<kTileThreads>(bc, bm, red);

@@ -571,7 +571,7 @@ def test_grid_16384_tile_orders_bitwise_n27_ref_loop():
     r = np.random.default_rng(27).uniform(0.2, 1.8, n + 1)
     x0 = _random_state(n, 270)
     outs = []
-    for opts in [dict(tiled=0)] + [dict(tiled=1, tile_blocks=16384, xcd_order=o) for o in range(4)]:
+    for opts in [dict(tiled=0)] + [dict(tiled=1, tile_blocks=16384, xcd_order=o) for o in range(5)]:
         with T.Engine(n, "f64", "ref", "mf") as e:
             for k, v in opts.items():
                 e.set_option(getattr(T, "OPT_" + k.upper()), v)
@@ -591,7 +591,7 @@ def test_grid_16384_tile_orders_n27_rk4_f64():
     r = np.random.default_rng(28).uniform(0.2, 1.8, n + 1)
     x0 = _random_state(n, 271)
     outs = []
-    for opts in [dict(tiled=0)] + [dict(tiled=1, tile_blocks=16384, xcd_order=o) for o in range(4)]:
+    for opts in [dict(tiled=0)] + [dict(tiled=1, tile_blocks=16384, xcd_order=o) for o in range(5)]:
         with T.Engine(n, "f64", "rk4", "mf") as e:
             for k, v in opts.items():
                 e.set_option(getattr(T, "OPT_" + k.upper()), v)
@@ -741,8 +741,8 @@ def test_xcd_order_does_not_change_results():
 
 @pytest.mark.parametrize("case", ["f32-horner", "f32-staged-tv", "f64-horner", "f64-ref"])
 def test_tile_orders_bitwise_equal(case):
-    """Every tile order (grid, XCD eighths, per-XCD sweep, device sweep),
-    grid size and output store policy gives the same bits: each state's row
+    """Every tile order (grid, XCD eighths, per-XCD sweep, device sweep,
+    level walk), grid size and output store policy gives the same bits: each state's row
     is summed in a fixed order whichever workgroup owns its tile.  n=22 with
     256 workgroups (also the default grid there) gives every workgroup
     several tiles, so the sweeps are really taken; 1024 gives each FP64
@@ -755,7 +755,8 @@ def test_tile_orders_bitwise_equal(case):
               period=rng.uniform(0.5, 2, n + 1), offset=rng.uniform(0, 1, n + 1), c=np.ones(n + 1))
     out = []
     for order, blocks, store in ((1, 0, 0), (0, 256, 0), (1, 256, 0), (2, 256, 0), (3, 256, 0), (2, 0, 0),
-                                 (3, 0, 0), (2, 0, 1), (2, 0, 2), (2, 1024, 0), (1, 1024, 0)):
+                                 (3, 0, 0), (2, 0, 1), (2, 0, 2), (2, 1024, 0), (1, 1024, 0), (4, 0, 0),
+                                 (4, 256, 0), (4, 1024, 0), (4, 0, 1)):
         with T.Engine(n, prec, "ref" if kind == "ref" else "rk4", "mf") as e:
             e.set_option(T.OPT_XCD_ORDER, order)
             e.set_option(T.OPT_TILE_BLOCKS, blocks)
@@ -823,7 +824,8 @@ def test_tiled_ref_bit_exact_vs_per_state_kernel(n):
     a, ia = _ref_run(n, r, 30, x0, tiled=2)
     b, ib = _ref_run(n, r, 30, x0, tiled=0)
     c, _ = _ref_run(n, r, 30, x0, tiled=2, xcd_order=0, tile_blocks=64)
-    assert _bits_equal(a, b) and _bits_equal(a, c)
+    d, _ = _ref_run(n, r, 30, x0, tiled=2, xcd_order=4)
+    assert _bits_equal(a, b) and _bits_equal(a, c) and _bits_equal(a, d)
     assert ia.t == ib.t and abs(ia.last_checksum - ib.last_checksum) <= 1e-14 * ib.last_checksum
 
 
