@@ -534,23 +534,21 @@ static int device_cus() {
     return v > 0 ? v : 0;
 }
 
-// Grid and tile order of a tiled pass.  Grid: 16 workgroups per resident
-// slot, which balances the tail (measured at n=28: 0.92 vs 1.03 ms for one
-// resident wave), at most kMaxBlocks partials for the end-of-step
+// Grid and tile order of a tiled pass.  Grid: 4 workgroups per resident
+// slot (1024 on 256 CUs), at most kMaxBlocks partials for the end-of-step
 // reduction, rounded down to a multiple of 8 XCDs; and at most ntiles/8
 // workgroups but no fewer than one per CU, so that where tiles are few
 // (mid n) each workgroup still walks several and the next tile's reads
 // overlap the current tile's LDS phase (per step, by grid: reference loop
 // FP64 n=22 / 23 / 24 24.3 / 44.9 / 89.3 us against 30.2 / 59.5 / 116.0
 // with one tile per workgroup; RK4 FP64 n=22..24 and FP32 n=23..25 12-16%
-// faster; within 3% of the best grid for n=21..27, unchanged at n >= 28).
-// The RK4 passes (`wide`)
-// take the largest power of two from 8192 up to kMaxBlocks workgroups that
-// still gives each 8 tiles or more (per step, 8192 vs 4096 workgroups: FP64
-// n=28 6.84 vs 6.92 ms, n=30 32.7 vs 33.7 ms, FP32 n=30 15.5 vs 15.6 ms;
-// 16384 vs 8192 at FP32 n=32: 69.4-71.6 vs 72.4-74.2 ms; with 4 tiles each,
-// FP32 n=28 and FP64 n=27, 0.6-2.4% slower; the reference loop at n=28
-// 0.9% slower).  Order (tile_walk): the requested one where the
+// faster).  On the level walk, per step with 1024 / 2048 / 4096 / 8192 /
+// 16384 workgroups: RK4 FP32 n=28 3.056 / 3.070 / 3.105 / 3.162 / 3.252 ms,
+// reference loop n=28 1.513 / 1.521 / 1.538 / 1.571 / 1.638 ms, RK4 FP64
+// n=28 6.107 / 6.098 / 6.100 / 6.127 / 6.223 ms; n=30 within 1% for every
+// grid.  The RK4 passes (`wide`) over 2^18 tiles or more (FP32 n >= 31,
+// FP64 n >= 30) take kMaxBlocks workgroups (FP32 n=32: 60.2 ms per step
+// against 61.7 with 4096).  Order (tile_walk): the requested one where the
 // grid allows it, else the next simpler one.
 struct TileLaunch {
     uint64_t grid;
@@ -614,13 +612,9 @@ static TileLaunch tile_launch(tasep_ctx *c, const void *kernel, int &slots_cache
                               bool wide = false) {
     if (!slots_cache) slots_cache = resident_slots(kernel);
     const int slots = slots_cache;
-    uint64_t cap = slots > 0 ? ((uint64_t)slots * 16) & ~(uint64_t)7 : 1024;
+    uint64_t cap = slots > 0 ? ((uint64_t)slots * 4) & ~(uint64_t)7 : 1024;
     if (cap < 8) cap = 8;
-    if (wide) {
-        uint64_t w = 8192;
-        while (w * 2 <= (uint64_t)kMaxBlocks && ntiles >= w * 2 * 8) w *= 2;
-        if (ntiles >= w * 8) cap = w;
-    }
+    if (wide && ntiles >= ((uint64_t)1 << 18)) cap = kMaxBlocks;
     if (cap > (uint64_t)kMaxBlocks) cap = kMaxBlocks;
     const int cus = device_cus();
     uint64_t few = ntiles / 8;

@@ -60,6 +60,11 @@ constexpr int kTileWaves = kTileThreads / kWave;
 #ifndef TASEP_TILE_SLOTS
 #define TASEP_TILE_SLOTS 2
 #endif
+// The same for passes whose self term is the gathered vector (no separate
+// self stream, fewer registers).
+#ifndef TASEP_TILE_SLOTS_SELF
+#define TASEP_TILE_SLOTS_SELF 2
+#endif
 // The same for the reference-loop tiles.
 #ifndef TASEP_REF_SLOTS
 #define TASEP_REF_SLOTS 1
@@ -281,7 +286,7 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
     const bool use_self = (EPI != EPI_STORE) || (!SELF && a.use_self);
     Kahan chk, mass;
 
-    constexpr int NS = TASEP_TILE_SLOTS;
+    constexpr int NS = SELF ? TASEP_TILE_SLOTS_SELF : TASEP_TILE_SLOTS;
     using Str = Cols<V, C, E>;
     Str gn;                                                        // own vector of the next tile
     uint64_t T = tile_at(a.walk, P0);
