@@ -403,7 +403,7 @@ __global__ __launch_bounds__(kBatchTileThreads, 8) void k_batch_tile(const Batch
 
 // ===========================================================================
 // Big-tile form of the batched pass: the single-lattice tile pass
-// (tile_pass_tiles: 512 lanes, tiles of 2^13 FP32 / 2^12 FP64 states with the
+// (tile_pass_from: 512 lanes, tiles of 2^13 FP32 / 2^12 FP64 states with the
 // top three state bits in the tile, every global read of a tile issued before
 // its LDS phase and the next tile's own vector prefetched) over a range of
 // consecutive tiles of one instance per workgroup.  Needs n >= 14 (FP32) or
@@ -491,16 +491,16 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_batch_bigtil
         // Horner: u = x + h/4 Lx, v = x + h/3 Lu, u = x + h/2 Lv, x += h Lu
         if constexpr (P == 0) {
             a.gx = x; a.out = u; a.c = (V)(0.25 * h);
-            tile_pass_from<V, EPI_STORE, true, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
+            tile_pass_from<V, EPI_STORE, true, NT>(a, rr, tile, red, WalkTiles(nullptr, r.t0, 1, r.count), r.part, first);
         } else if constexpr (P == 1) {
             a.gx = u; a.out = v; a.c = (V)(h / 3.0);
-            tile_pass_from<V, EPI_STORE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
+            tile_pass_from<V, EPI_STORE, false, NT>(a, rr, tile, red, WalkTiles(nullptr, r.t0, 1, r.count), r.part, first);
         } else if constexpr (P == 2) {
             a.gx = v; a.out = u; a.c = (V)(0.5 * h);
-            tile_pass_from<V, EPI_STORE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
+            tile_pass_from<V, EPI_STORE, false, NT>(a, rr, tile, red, WalkTiles(nullptr, r.t0, 1, r.count), r.part, first);
         } else {
             a.gx = u; a.out = x; a.c = (V)h;
-            tile_pass_from<V, EPI_HORNER, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
+            tile_pass_from<V, EPI_HORNER, false, NT>(a, rr, tile, red, WalkTiles(nullptr, r.t0, 1, r.count), r.part, first);
         }
     } else {
         V *kacc = b.acc + in.off;
@@ -509,16 +509,16 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_batch_bigtil
         a.acc_out = kacc;
         if constexpr (P == 0) {
             a.gx = x; a.out = u; a.cy = (V)0.5; a.stage = 0;
-            tile_pass_from<V, EPI_STAGE, true, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
+            tile_pass_from<V, EPI_STAGE, true, NT>(a, rr, tile, red, WalkTiles(nullptr, r.t0, 1, r.count), r.part, first);
         } else if constexpr (P == 1) {
             a.gx = u; a.out = v; a.cy = (V)0.5; a.stage = 1;
-            tile_pass_from<V, EPI_STAGE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
+            tile_pass_from<V, EPI_STAGE, false, NT>(a, rr, tile, red, WalkTiles(nullptr, r.t0, 1, r.count), r.part, first);
         } else if constexpr (P == 2) {
             a.gx = v; a.out = u; a.cy = (V)1.0; a.stage = 2;
-            tile_pass_from<V, EPI_STAGE, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
+            tile_pass_from<V, EPI_STAGE, false, NT>(a, rr, tile, red, WalkTiles(nullptr, r.t0, 1, r.count), r.part, first);
         } else {
             a.gx = u; a.out = x;
-            tile_pass_from<V, EPI_RK4ACC, false, NT>(a, rr, tile, red, r.t0, 1, r.count, r.part, first);
+            tile_pass_from<V, EPI_RK4ACC, false, NT>(a, rr, tile, red, WalkTiles(nullptr, r.t0, 1, r.count), r.part, first);
         }
     }
 }

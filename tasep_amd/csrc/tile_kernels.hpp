@@ -69,6 +69,12 @@ constexpr int kTileWaves = kTileThreads / kWave;
 #ifndef TASEP_REF_SLOTS
 #define TASEP_REF_SLOTS 1
 #endif
+// Tile bonds beyond the pre-issued ones: 0 = two in flight at a time, each
+// consumed as the next is issued; B > 0 = issued B at a time, all in
+// flight together, then consumed in ascending order.
+#ifndef TASEP_DYN_BATCH
+#define TASEP_DYN_BATCH 0
+#endif
 
 template <class V> struct TileGeom;
 // E states per lane per column (one 16-byte load), C = 4 columns per lane.
@@ -208,9 +214,39 @@ __device__ __forceinline__ void load_cols_x(const V *p, uint64_t base, uint32_t 
 struct TileWalk {
     uint64_t base, step;
 };
-__device__ __forceinline__ uint64_t tile_at(const uint32_t *walk, uint64_t p) {
-    return walk ? (uint64_t)walk[p] : p;
-}
+// Tiles of one workgroup's positions P0 + k * step, k < count.  With a
+// level-walk table, lane i of every wave holds the tile of position k0 + i,
+// loaded once per chunk of 63 tiles (64 entries: the chunk's tiles and the
+// next one, which a tile prefetches), and a tile is read with readlane.
+// Callers walk a chunk in an inner loop with no load of the table inside
+// it: a table load per tile would make the wave wait for all its
+// outstanding memory operations (the previous tile's output stores, the
+// prefetched next tile) at every tile.
+struct WalkTiles {
+    static constexpr uint64_t kChunk = 63;
+    const uint32_t *walk;
+    uint64_t P0, step, count, k0;
+    uint32_t v;
+    __device__ __forceinline__ WalkTiles(const uint32_t *w, uint64_t p0, uint64_t st, uint64_t cnt)
+        : walk(w), P0(p0), step(st), count(cnt), k0(0), v(0) {
+        if (walk) load(0);
+    }
+    __device__ __forceinline__ void load(uint64_t k) {
+        k0 = k;
+        if (!walk) return;
+        const uint64_t kl = k + (threadIdx.x & 63);
+        const uint32_t raw = kl < count ? walk[P0 + kl * step] : 0u;
+        // copied once by a lane-identity DPP move: the wait for the load
+        // happens here, and the loop reads a register with no load pending
+        v = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)raw, 0xE4, 0xF, 0xF, false);
+    }
+    // position k of the loaded chunk: k0 <= k <= k0 + 63
+    __device__ __forceinline__ uint64_t at(uint64_t k) const {
+        if (!walk) return P0 + k * step;
+        return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)v, (int)(k - k0));
+    }
+};
+
 __device__ __forceinline__ TileWalk tile_walk(uint64_t ntiles, int order, uint32_t res) {
     const uint64_t g = blockIdx.x, G = gridDim.x;
     if (order == 0) return {g, G};
@@ -233,16 +269,16 @@ __device__ __forceinline__ int pop_bond(uint64_t &m, int LB0) {
     return k;
 }
 
-// One workgroup's share of a tiled pass: the tiles at walk positions P0,
-// P0 + step, ... (`count` of them) of the lattice described by `a`, with the pass's rates already in LDS
+// One workgroup's share of a tiled pass: the tiles at its walk positions
+// (`tiles`) of the lattice described by `a`, with the pass's rates already in LDS
 // (rr) and the tile and reduction buffers supplied by the kernel.  The
 // step's final passes leave the workgroup's (sum |dx|, sum x) in partial
 // slot `part`.  `cur` holds the first tile's own vector, already issued by
-// the caller (tile_pass_tiles below issues it).
+// the caller (the kernel issues it before its prologue waits).
 template <class V, int EPI, bool SELF, bool NT>
 __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__restrict__ rr, V *tile,
-                                               double *red, uint64_t P0, uint64_t step, uint64_t count,
-                                               uint64_t part, Cols<V, TileGeom<V>::C, TileGeom<V>::E> cur) {
+                                               double *red, WalkTiles tiles, uint64_t part,
+                                               Cols<V, TileGeom<V>::C, TileGeom<V>::E> cur) {
     using G = TileGeom<V>;
     constexpr int E = G::E, e = G::e, LB = G::LB, C = G::C, TB0 = G::TB0;
     const int n = a.n;
@@ -289,11 +325,13 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
     constexpr int NS = SELF ? TASEP_TILE_SLOTS_SELF : TASEP_TILE_SLOTS;
     using Str = Cols<V, C, E>;
     Str gn;                                                        // own vector of the next tile
-    uint64_t T = tile_at(a.walk, P0);
-    uint64_t Tn = count > 1 ? tile_at(a.walk, P0 + step) : 0;
-    for (uint64_t kt = 0; kt < count; kt++) {
-        // the tile after next, read a whole tile ahead of its use
-        const uint64_t Tnn = kt + 2 < count ? tile_at(a.walk, P0 + (kt + 2) * step) : 0;
+    const uint64_t count = tiles.count;
+    for (uint64_t c0 = 0; c0 < count; c0 += WalkTiles::kChunk) {
+      if (c0) tiles.load(c0);
+      const uint64_t cend = c0 + WalkTiles::kChunk < count ? c0 + WalkTiles::kChunk : count;
+      for (uint64_t kt = c0; kt < cend; kt++) {
+        const uint64_t T = tiles.at(kt);
+        const uint64_t Tn = kt + 1 < count ? tiles.at(kt + 1) : 0;
         const uint64_t wb = (T << TB0) + wofs;                     // wave base (uniform)
         const bool t_lo = T & 1, t_hi = (T >> (tb - 1)) & 1;
         // ---- issue every global read of this tile in the order it is used:
@@ -414,6 +452,22 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
         for (int q = 0; q < NS; q++)
             if (ks[q] >= 0) fma_cols<V, E, C, C>(rr[n - 1 - ks[q]], S[q], acc);
         if (NS == 0 || ks[NS - 1] >= 0) {
+#if TASEP_DYN_BATCH > 0
+            for (;;) {
+                constexpr int B = TASEP_DYN_BATCH;
+                Str D[B];
+                int kd[B];
+#pragma unroll
+                for (int q = 0; q < B; q++) {
+                    kd[q] = pop_bond(mm, TB0);
+                    if (kd[q] >= 0) issue_cols<V, E, C, LB>(a, wb + ((uint64_t)1 << kd[q]), lo, D[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < B; q++)
+                    if (kd[q] >= 0) fma_cols<V, E, C, C>(rr[n - 1 - kd[q]], D[q], acc);
+                if (kd[B - 1] < 0 || !mm) break;
+            }
+#else
             int kA = pop_bond(mm, TB0);
             if (kA >= 0) issue_cols<V, E, C, LB>(a, wb + ((uint64_t)1 << kA), lo, S[0]);
             while (kA >= 0) {
@@ -425,6 +479,7 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
                 if (kA >= 0) issue_cols<V, E, C, LB>(a, wb + ((uint64_t)1 << kA), lo, S[0]);
                 fma_cols<V, E, C, C>(rr[n - 1 - kB], S[1], acc);
             }
+#endif
         }
         // ---- diagonal and epilogue
 #pragma unroll
@@ -487,8 +542,7 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
             st_out<NT>(a.out + wb + lo + ((uint32_t)c << LB), o);
         }
         if (more) cur = gn;
-        T = Tn;
-        Tn = Tnn;
+      }
     }
     if constexpr (EPI == EPI_HORNER || EPI == EPI_RK4ACC) {
         double bc = chk.value(), bm = mass.value();
@@ -498,21 +552,6 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
             a.pmass[part] = bm;
         }
     }
-}
-
-// The same, issuing the first tile's own vector itself.
-template <class V, int EPI, bool SELF, bool NT>
-__device__ __forceinline__ void tile_pass_tiles(const TileArgs<V> &a, const V *__restrict__ rr, V *tile,
-                                                double *red, uint64_t P0, uint64_t step, uint64_t count,
-                                                uint64_t part) {
-    using G = TileGeom<V>;
-    Cols<V, G::C, G::E> cur;
-    if (count > 0) {
-        const uint64_t wofs = (uint64_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) << (a.n - 3);
-        issue_cols<V, G::E, G::C, G::LB>(a, (tile_at(a.walk, P0) << G::TB0) + wofs,
-                                         (uint32_t)(threadIdx.x & 63) * G::E, cur);
-    }
-    tile_pass_from<V, EPI, SELF, NT>(a, rr, tile, red, P0, step, count, part, cur);
 }
 
 template <class V, int EPI, bool SELF, bool NT>
@@ -530,16 +569,16 @@ __global__ __launch_bounds__(kTileThreads, TASEP_TILE_WAVES) void k_tile_pass(co
     // anything waits (the stopped flag is a scalar load)
     const int t = threadIdx.x;
     const double rate = t <= a.n ? a.rates[a.rset * kRateStride + t] : 0.0;
+    WalkTiles tiles(a.walk, walk.base, walk.step, count);
     Cols<V, TG::C, TG::E> first;
     if (count > 0) {
         const uint64_t wofs = (uint64_t)__builtin_amdgcn_readfirstlane(t >> 6) << (a.n - 3);
-        issue_cols<V, TG::E, TG::C, TG::LB>(a, (tile_at(a.walk, walk.base) << TG::TB0) + wofs,
-                                            (uint32_t)(t & 63) * TG::E, first);
+        issue_cols<V, TG::E, TG::C, TG::LB>(a, (tiles.at(0) << TG::TB0) + wofs, (uint32_t)(t & 63) * TG::E, first);
     }
     if (a.st->stopped) return;
     if (t <= a.n) rr[t] = (V)rate;
     __syncthreads();
-    tile_pass_from<V, EPI, SELF, NT>(a, rr, tile, red, walk.base, walk.step, count, g, first);
+    tile_pass_from<V, EPI, SELF, NT>(a, rr, tile, red, tiles, g, first);
 }
 
 // ===========================================================================
@@ -595,10 +634,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
     // before anything waits (the stopped flag is a scalar load), so that a
     // workgroup with a single tile (mid n) pays one memory latency, not three.
     const double rate = t <= n ? a.rates[t] : 0.0;
-    uint64_t it = blockIdx.x, kt = 0;
+    const uint64_t g0 = blockIdx.x;
+    uint64_t kt = 0;
     double g[C][E];
-    uint64_t T = it < a.ntiles ? tile_at(a.walk, walk.base) : 0;
-    if (it < a.ntiles) issue((T << TB0) + wofs, g);
+    const uint64_t count = g0 < a.ntiles ? (a.ntiles - g0 + gridDim.x - 1) / gridDim.x : 0;
+    WalkTiles tiles(a.walk, walk.base, walk.step, count);
+    if (count) issue((tiles.at(0) << TB0) + wofs, g);
     if (a.st->stopped) return;
     if (t <= n) rr[t] = rate;
     __syncthreads();
@@ -639,16 +680,17 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
     const double w_b6 = lane_hi ? rb(6) : 0.0;                    // bond 6 in-flow, columns 0, 2
     const double d_b6 = lane_hi ? 0.0 : rb(6);                    // bond 6 out-flow, columns 1, 3
     Kahan chk, mass;
-    const uint64_t stride = gridDim.x;
     const double h = a.h;
 
     // One tile bond pre-issued (bond n-4 first): FP64 n=20..26 3-5% faster
     // per step than two, the same at n=28; three are 3% slower.
     constexpr int NS = TASEP_REF_SLOTS;
-    uint64_t Tn = it + stride < a.ntiles ? tile_at(a.walk, walk.base + walk.step) : 0;
-    for (; it < a.ntiles; it += stride, kt++) {
-        // the tile after next, read a whole tile ahead of its use
-        const uint64_t Tnn = it + 2 * stride < a.ntiles ? tile_at(a.walk, walk.base + (kt + 2) * walk.step) : 0;
+    for (uint64_t c0 = 0; c0 < count; c0 += WalkTiles::kChunk) {
+      if (c0) tiles.load(c0);
+      const uint64_t cend = c0 + WalkTiles::kChunk < count ? c0 + WalkTiles::kChunk : count;
+      for (kt = c0; kt < cend; kt++) {
+        const uint64_t T = tiles.at(kt);
+        const uint64_t Tn = kt + 1 < count ? tiles.at(kt + 1) : 0;
         const uint64_t wb = (T << TB0) + wofs;
         const bool t_lo = T & 1, t_hi = (T >> (tb - 1)) & 1;
         // ---- issue every global read of this tile before its LDS phase:
@@ -671,7 +713,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
             if (ks[q] >= 0) issue(wb + ((uint64_t)1 << ks[q]), S[q]);
         }
         double gn[C][E];
-        const bool more = it + stride < a.ntiles;
+        const bool more = kt + 1 < count;
         if (more) issue((Tn << TB0) + wofs, gn);
         __syncthreads();  // the previous tile's LDS reads are done
 #pragma unroll
@@ -773,6 +815,22 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
         for (int q = 0; q < NS; q++)
             if (ks[q] >= 0) add(ks[q], S[q]);
         if (ks[NS - 1] >= 0) {
+#if TASEP_DYN_BATCH > 0
+            for (;;) {
+                constexpr int B = TASEP_DYN_BATCH;
+                double D[B][C][E];
+                int kd[B];
+#pragma unroll
+                for (int q = 0; q < B; q++) {
+                    kd[q] = pop_bond(mm, TB0);
+                    if (kd[q] >= 0) issue(wb + ((uint64_t)1 << kd[q]), D[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < B; q++)
+                    if (kd[q] >= 0) add(kd[q], D[q]);
+                if (kd[B - 1] < 0 || !mm) break;
+            }
+#else
             int kA = pop_bond(mm, TB0);
             if (kA >= 0) issue(wb + ((uint64_t)1 << kA), S[0]);
             while (kA >= 0) {
@@ -784,6 +842,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
                 if (kA >= 0) issue(wb + ((uint64_t)1 << kA), S[0]);
                 add(kB, S[1]);
             }
+#endif
         }
         // ---- bond n-4: chunk w+1 of tile T - 2^(tb-1), 2^(n-4) states away
         if (has_n4) add(n - 4, s4);
@@ -830,8 +889,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
 #pragma unroll
                 for (int j = 0; j < E; j++) g[c][j] = gn[c][j];
         }
-        T = Tn;
-        Tn = Tnn;
+      }
     }
     double bc = chk.value(), bm = mass.value();
     block_sum2<kTileThreads>(bc, bm, red);
