@@ -65,6 +65,16 @@ constexpr int kTileWaves = kTileThreads / kWave;
 #ifndef TASEP_TILE_SLOTS_SELF
 #define TASEP_TILE_SLOTS_SELF 2
 #endif
+// Order within a tile: 1 = the prefetched own vector goes to LDS before the
+// tile's streams are issued, 0 = after, for FP32 and FP64 tiles.  Measured
+// per step at n=28 (first / after): RK4 FP32 2.89 / 3.07 ms (P2/P3 0.741 /
+// 0.783 ms), RK4 FP64 6.10 / 6.02 ms, reference loop 1.634 / 1.586 ms.
+#ifndef TASEP_OWN_FIRST_F32
+#define TASEP_OWN_FIRST_F32 1
+#endif
+#ifndef TASEP_OWN_FIRST_F64
+#define TASEP_OWN_FIRST_F64 0
+#endif
 // The same for the reference-loop tiles.
 #ifndef TASEP_REF_SLOTS
 #define TASEP_REF_SLOTS 1
@@ -334,6 +344,18 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
         const uint64_t Tn = kt + 1 < count ? tiles.at(kt + 1) : 0;
         const uint64_t wb = (T << TB0) + wofs;                     // wave base (uniform)
         const bool t_lo = T & 1, t_hi = (T >> (tb - 1)) & 1;
+        constexpr bool kOwnFirst = sizeof(V) == 4 ? TASEP_OWN_FIRST_F32 : TASEP_OWN_FIRST_F64;
+        const V(&g)[C][E] = cur.g;
+        if constexpr (kOwnFirst) {
+            // ---- own vector to LDS first.  It was prefetched a tile ago;
+            // the reads below are issued after it, so the wait for it (vmcnt
+            // counts in issue order) covers only the own vector and the
+            // previous tile's stores, not this tile's conditionally issued
+            // streams, and the LDS phase overlaps their latency.
+            __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+            for (int c = 0; c < C; c++) st16(&tile[lds0 + c * LCOL], g[c]);
+        }
         // ---- issue every global read of this tile in the order it is used:
         // bond LB+1 (columns 2, 3 read columns 0, 1 of tile T+1), bond n-4
         // (per wave), the first NS tile bonds, the self vector, then the next
@@ -360,12 +382,12 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
         }
         const bool more = kt + 1 < count;
         if (more) issue_cols<V, E, C, LB>(a, (Tn << TB0) + wofs, lo, gn);
-        // ---- own vector to LDS (waited for only here: the first tile's
-        // streams are issued before it has arrived)
-        const V(&g)[C][E] = cur.g;
-        __syncthreads();  // the previous tile's LDS reads are done
+        if constexpr (!kOwnFirst) {
+            // ---- own vector to LDS (waited for only here)
+            __syncthreads();  // the previous tile's LDS reads are done
 #pragma unroll
-        for (int c = 0; c < C; c++) st16(&tile[lds0 + c * LCOL], g[c]);
+            for (int c = 0; c < C; c++) st16(&tile[lds0 + c * LCOL], g[c]);
+        }
         // ---- uniform and per-wave parts of the diagonal
         const uint64_t hout = ~T & (T >> 1) & tmask;
         V d_t = d_fix;
@@ -693,6 +715,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
         const uint64_t Tn = kt + 1 < count ? tiles.at(kt + 1) : 0;
         const uint64_t wb = (T << TB0) + wofs;
         const bool t_lo = T & 1, t_hi = (T >> (tb - 1)) & 1;
+#if TASEP_OWN_FIRST_F64
+        // ---- own vector to LDS first (prefetched a tile ago; waiting for it
+        // then does not wait for this tile's streams, issued below)
+        __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+        for (int c = 0; c < C; c++) st16(&tile[lds0 + c * LCOL], g[c]);
+#endif
         // ---- issue every global read of this tile before its LDS phase:
         // bond 8 (columns 2, 3 read columns 0, 1 of tile T+1), bond n-4, the
         // first NS tile bonds, then the next tile's own vector
@@ -715,9 +744,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
         double gn[C][E];
         const bool more = kt + 1 < count;
         if (more) issue((Tn << TB0) + wofs, gn);
+#if !TASEP_OWN_FIRST_F64
         __syncthreads();  // the previous tile's LDS reads are done
 #pragma unroll
         for (int c = 0; c < C; c++) st16(&tile[lds0 + c * LCOL], g[c]);
+#endif
         __syncthreads();
         const uint64_t hout = ~T & (T >> 1) & tmask;
         double Tv[C][E];
