@@ -60,11 +60,6 @@ constexpr int kTileWaves = kTileThreads / kWave;
 #ifndef TASEP_TILE_SLOTS
 #define TASEP_TILE_SLOTS 2
 #endif
-// The same for passes whose self term is the gathered vector (no separate
-// self stream, fewer registers).
-#ifndef TASEP_TILE_SLOTS_SELF
-#define TASEP_TILE_SLOTS_SELF 2
-#endif
 // Order within a tile: 1 = the prefetched own vector goes to LDS before the
 // tile's streams are issued, 0 = after, for FP32 and FP64 tiles.  Measured
 // per step at n=28 (first / after): RK4 FP32 2.89 / 3.07 ms (P2/P3 0.741 /
@@ -78,12 +73,6 @@ constexpr int kTileWaves = kTileThreads / kWave;
 // The same for the reference-loop tiles.
 #ifndef TASEP_REF_SLOTS
 #define TASEP_REF_SLOTS 1
-#endif
-// Tile bonds beyond the pre-issued ones: 0 = two in flight at a time, each
-// consumed as the next is issued; B > 0 = issued B at a time, all in
-// flight together, then consumed in ascending order.
-#ifndef TASEP_DYN_BATCH
-#define TASEP_DYN_BATCH 0
 #endif
 
 template <class V> struct TileGeom;
@@ -332,7 +321,7 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
     const bool use_self = (EPI != EPI_STORE) || (!SELF && a.use_self);
     Kahan chk, mass;
 
-    constexpr int NS = SELF ? TASEP_TILE_SLOTS_SELF : TASEP_TILE_SLOTS;
+    constexpr int NS = TASEP_TILE_SLOTS;
     using Str = Cols<V, C, E>;
     Str gn;                                                        // own vector of the next tile
     const uint64_t count = tiles.count;
@@ -381,7 +370,11 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
             if (EPI == EPI_RK4ACC || a.stage > 0) load_cols_x<V, E, C, LB>(a.acc_in, wb, lo, ac);
         }
         const bool more = kt + 1 < count;
-        if (more) issue_cols<V, E, C, LB>(a, (Tn << TB0) + wofs, lo, gn);
+        // issued on every tile (the last re-reads its own vector, an L2
+        // hit) so that the compiler can count it: a wait for an earlier
+        // stream then leaves it in flight (vmcnt is in issue order, and a
+        // conditional load is counted as absent)
+        issue_cols<V, E, C, LB>(a, ((more ? Tn : T) << TB0) + wofs, lo, gn);
         if constexpr (!kOwnFirst) {
             // ---- own vector to LDS (waited for only here)
             __syncthreads();  // the previous tile's LDS reads are done
@@ -474,22 +467,6 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
         for (int q = 0; q < NS; q++)
             if (ks[q] >= 0) fma_cols<V, E, C, C>(rr[n - 1 - ks[q]], S[q], acc);
         if (NS == 0 || ks[NS - 1] >= 0) {
-#if TASEP_DYN_BATCH > 0
-            for (;;) {
-                constexpr int B = TASEP_DYN_BATCH;
-                Str D[B];
-                int kd[B];
-#pragma unroll
-                for (int q = 0; q < B; q++) {
-                    kd[q] = pop_bond(mm, TB0);
-                    if (kd[q] >= 0) issue_cols<V, E, C, LB>(a, wb + ((uint64_t)1 << kd[q]), lo, D[q]);
-                }
-#pragma unroll
-                for (int q = 0; q < B; q++)
-                    if (kd[q] >= 0) fma_cols<V, E, C, C>(rr[n - 1 - kd[q]], D[q], acc);
-                if (kd[B - 1] < 0 || !mm) break;
-            }
-#else
             int kA = pop_bond(mm, TB0);
             if (kA >= 0) issue_cols<V, E, C, LB>(a, wb + ((uint64_t)1 << kA), lo, S[0]);
             while (kA >= 0) {
@@ -501,7 +478,6 @@ __device__ __forceinline__ void tile_pass_from(const TileArgs<V> &a, const V *__
                 if (kA >= 0) issue_cols<V, E, C, LB>(a, wb + ((uint64_t)1 << kA), lo, S[0]);
                 fma_cols<V, E, C, C>(rr[n - 1 - kB], S[1], acc);
             }
-#endif
         }
         // ---- diagonal and epilogue
 #pragma unroll
@@ -743,7 +719,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
         }
         double gn[C][E];
         const bool more = kt + 1 < count;
-        if (more) issue((Tn << TB0) + wofs, gn);
+        // issued on every tile (the last re-reads its own vector) so that
+        // the compiler can count it: waits for earlier streams leave it in
+        // flight
+        issue(((more ? Tn : T) << TB0) + wofs, gn);
 #if !TASEP_OWN_FIRST_F64
         __syncthreads();  // the previous tile's LDS reads are done
 #pragma unroll
@@ -846,22 +825,6 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
         for (int q = 0; q < NS; q++)
             if (ks[q] >= 0) add(ks[q], S[q]);
         if (ks[NS - 1] >= 0) {
-#if TASEP_DYN_BATCH > 0
-            for (;;) {
-                constexpr int B = TASEP_DYN_BATCH;
-                double D[B][C][E];
-                int kd[B];
-#pragma unroll
-                for (int q = 0; q < B; q++) {
-                    kd[q] = pop_bond(mm, TB0);
-                    if (kd[q] >= 0) issue(wb + ((uint64_t)1 << kd[q]), D[q]);
-                }
-#pragma unroll
-                for (int q = 0; q < B; q++)
-                    if (kd[q] >= 0) add(kd[q], D[q]);
-                if (kd[B - 1] < 0 || !mm) break;
-            }
-#else
             int kA = pop_bond(mm, TB0);
             if (kA >= 0) issue(wb + ((uint64_t)1 << kA), S[0]);
             while (kA >= 0) {
@@ -873,7 +836,6 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
                 if (kA >= 0) issue(wb + ((uint64_t)1 << kA), S[0]);
                 add(kB, S[1]);
             }
-#endif
         }
         // ---- bond n-4: chunk w+1 of tile T - 2^(tb-1), 2^(n-4) states away
         if (has_n4) add(n - 4, s4);
