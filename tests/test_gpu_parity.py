@@ -604,6 +604,30 @@ def test_grid_16384_tile_orders_n27_rk4_f64():
     assert float(np.max(np.abs(outs[1] - outs[0]) / np.abs(outs[0]))) <= 1e-13
 
 
+@pytest.mark.parametrize("prec,integ", [("f64", "ref"), ("f32", "rk4")])
+def test_level_walk_table_chunks_n27(prec, integ):
+    """The level walk with 256 workgroups at n=27: 128 FP64 / 64 FP32 tiles
+    per workgroup, so each workgroup reloads its part of the walk table once
+    or twice (a chunk holds 63 tiles).  From a random normalised state the
+    result equals the grid order's bit for bit: a tile taken twice or
+    skipped would leave a stale or doubled slice."""
+    n = 27
+    r = np.random.default_rng(2727).uniform(0.2, 1.8, n + 1)
+    x0 = _random_state(n, 272)
+    outs = []
+    for order, blocks in ((0, 256), (4, 256)):
+        with T.Engine(n, prec, integ, "mf") as e:
+            e.set_option(T.OPT_TILED, 1)
+            e.set_option(T.OPT_TILE_BLOCKS, blocks)
+            e.set_option(T.OPT_XCD_ORDER, order)
+            e.set_rates(r)
+            e.set_x(x0)
+            e.run(0.0, 0.01, 2, patience=-1)
+            outs.append(e.get_x())
+    assert _bits_equal(outs[0], outs[1])
+    assert not _bits_equal(outs[1], x0)
+
+
 def test_full_size_n28_ref_tiles_bitwise_vs_per_state_kernel():
     """The bit-exact reference loop at the n=28 benchmark size: the tiled
     kernel (per-XCD sweep, pre-issued streams, hoisted diagonal, division by
