@@ -156,4 +156,38 @@ __device__ __forceinline__ void stage_rates(double *lds, const double *rates,
     __syncthreads();
 }
 
+// Tiles (or row groups) of one workgroup's positions P0 + k * step,
+// k < count.  With a level-walk table, lane i of every wave holds the tile
+// of position k0 + i,
+// loaded once per chunk of 63 tiles (64 entries: the chunk's tiles and the
+// next one, which a tile prefetches), and a tile is read with readlane.
+// Callers walk a chunk in an inner loop with no load of the table inside
+// it: a table load per tile would make the wave wait for all its
+// outstanding memory operations (the previous tile's output stores, the
+// prefetched next tile) at every tile.
+struct WalkTiles {
+    static constexpr uint64_t kChunk = 63;
+    const uint32_t *walk;
+    uint64_t P0, step, count, k0;
+    uint32_t v;
+    __device__ __forceinline__ WalkTiles(const uint32_t *w, uint64_t p0, uint64_t st, uint64_t cnt)
+        : walk(w), P0(p0), step(st), count(cnt), k0(0), v(0) {
+        if (walk) load(0);
+    }
+    __device__ __forceinline__ void load(uint64_t k) {
+        k0 = k;
+        if (!walk) return;
+        const uint64_t kl = k + (threadIdx.x & 63);
+        const uint32_t raw = kl < count ? walk[P0 + kl * step] : 0u;
+        // copied once by a lane-identity DPP move: the wait for the load
+        // happens here, and the loop reads a register with no load pending
+        v = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)raw, 0xE4, 0xF, 0xF, false);
+    }
+    // position k of the loaded chunk: k0 <= k <= k0 + 63
+    __device__ __forceinline__ uint64_t at(uint64_t k) const {
+        if (!walk) return P0 + k * step;
+        return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)v, (int)(k - k0));
+    }
+};
+
 }  // namespace tasep

@@ -25,6 +25,7 @@ struct RefArgs {
     uint64_t dim;
     double h;
     Lattice g;
+    const uint32_t *walk;  // CSR loop: level walk of the 64-row groups, or null
 };
 
 __global__ __launch_bounds__(kBlock) void k_ref_step(const RefArgs a) {
@@ -70,7 +71,24 @@ __global__ __launch_bounds__(kWave) void k_ref_csr_wave(const RefArgs a) {
     const int t = threadIdx.x;
     Kahan chk, mass;
     const uint64_t groups = (a.dim + kWave - 1) / kWave;
-    for (uint64_t gi = blockIdx.x; gi < groups; gi += gridDim.x) {
+    // Row groups in the level walk: the workgroups of XCD x (blockIdx % 8)
+    // take positions x * groups/8 ... in turn, so a group's far partners
+    // (whose x values it gathers) are walked close to it, as for the tiles.
+    // Without a table: grid order.
+    uint64_t P0 = blockIdx.x, step = gridDim.x;
+    if (a.walk) {
+        const uint64_t per = groups >> 3;
+        P0 = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+        step = gridDim.x >> 3;
+    }
+    const uint64_t lim = a.walk ? ((blockIdx.x & 7) + 1) * (groups >> 3) : groups;
+    const uint64_t count = P0 < lim ? (lim - P0 + step - 1) / step : 0;
+    WalkTiles tiles(a.walk, P0, step, count);
+    for (uint64_t c0 = 0; c0 < count; c0 += WalkTiles::kChunk) {
+      if (c0) tiles.load(c0);
+      const uint64_t cend_k = c0 + WalkTiles::kChunk < count ? c0 + WalkTiles::kChunk : count;
+      for (uint64_t kt = c0; kt < cend_k; kt++) {
+        const uint64_t gi = tiles.at(kt);
         const uint64_t s0 = gi * kWave, s = s0 + t;
         const uint64_t last = s0 + kWave < a.dim ? s0 + kWave : a.dim;
         const uint64_t seg0 = a.rowPtr[s0], seg1 = a.rowPtr[last];
@@ -121,6 +139,7 @@ __global__ __launch_bounds__(kWave) void k_ref_csr_wave(const RefArgs a) {
             chk.add(fabs(d));
             mass.add(xn);
         }
+      }
     }
     const double bc = wave_sum(chk.value());
     const double bm = wave_sum(mass.value());

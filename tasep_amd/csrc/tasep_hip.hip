@@ -759,6 +759,7 @@ static void ref_step(tasep_ctx *c, double h, uint64_t j) {
         return;
     }
     RefArgs a;
+    memset(&a, 0, sizeof a);
     a.x = (const double *)c->buf[c->cur ^ (int)(j & 1)];
     a.xn = (double *)c->buf[c->cur ^ (int)((j + 1) & 1)];
     a.rates = c->rates;
@@ -775,6 +776,10 @@ static void ref_step(tasep_ctx *c, double h, uint64_t j) {
     if (c->gen == TASEP_CSR) {
         const uint64_t groups = (c->dim + kWave - 1) / kWave;
         const uint32_t g = (uint32_t)(groups < (uint64_t)kMaxBlocks ? groups : kMaxBlocks);
+        // 64-row groups in the level walk over the group index (tile order 4;
+        // groups is a power of two >= 128 here, the grid a multiple of 8)
+        a.walk = c->xcd_order == 4 && groups >= 16 && (g % 8) == 0 ? ensure_walk(c, 63 - __builtin_clzll(groups))
+                                                                   : nullptr;
         hipLaunchKernelGGL(k_ref_csr_wave, dim3(g), dim3(kWave), 0, c->stream, a);
         prof_end(c, "ref_step_csr", 24.0 + 16.0 * (2.0 + (c->n - 1) / 4.0));
         launch_finish(c, g);
