@@ -550,6 +550,12 @@ static int device_cus() {
 // FP64 n >= 30) take kMaxBlocks workgroups (FP32 n=32: 60.2 ms per step
 // against 61.7 with 4096).  Order (tile_walk): the requested one where the
 // grid allows it, else the next simpler one.
+// The level-walk table the context's passes will use, built ahead of a
+// graph capture: its host-to-device copy cannot be captured.  (Built for
+// the tile geometry of the context's precision, or the 64-row groups of the
+// stored-matrix loop; a table the grid then cannot use is only memory.)
+static void prewarm_walk(tasep_ctx *c);
+
 struct TileLaunch {
     uint64_t grid;
     int order;                  // order the kernel walks its positions in (0..3)
@@ -597,9 +603,15 @@ static const uint32_t *ensure_walk(tasep_ctx *c, int tb) {
     if (c->walk_tb == tb) return c->walk;
     std::vector<uint32_t> w;
     level_walk(tb, w);
+    // without the table the passes take the order it permutes; clear the
+    // failed call's error so that it does not fail the run that follows
     uint32_t *d = nullptr;
-    if (hipMalloc(&d, w.size() * sizeof(uint32_t)) != hipSuccess) return nullptr;
+    if (hipMalloc(&d, w.size() * sizeof(uint32_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
     if (hipMemcpy(d, w.data(), w.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
         hipFree(d);
         return nullptr;
     }
@@ -714,6 +726,16 @@ static void rk4_staged_tile(tasep_ctx *c, double h, uint64_t j) {
     launch_tile<V, EPI_STAGE>(c, a, "rk4_s23", 5 * e);
     a.rset = 2; a.gx = ya; a.out = xn;
     launch_tile<V, EPI_RK4ACC>(c, a, "rk4_s4", 4 * e);
+}
+
+static void prewarm_walk(tasep_ctx *c) {
+    if (c->xcd_order != 4) return;
+    if (c->gen == TASEP_CSR) {
+        if (c->integ == TASEP_REF && c->n >= 10) ensure_walk(c, c->n - 6);
+        return;
+    }
+    const int b = c->prec == TASEP_F32 ? TileGeom<float>::b : TileGeom<double>::b;
+    if (c->n > b) ensure_walk(c, c->n - b);
 }
 
 // Tiles need n >= b + 1 (two tiles); below n = 18 the one-lane-per-state
@@ -998,6 +1020,7 @@ int tasep_run_resume(tasep_ctx *c, double t0, double h, uint64_t steps, double t
         hipGraph_t g = nullptr;
         hipGraphExec_t exec = nullptr;
         const uint32_t p0 = c->pending;
+        prewarm_walk(c);
         HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (uint64_t k = 0; k < chunk; k++) enqueue_step(c, h, k);
         hipError_t e = hipStreamEndCapture(c->stream, &g);

@@ -863,6 +863,27 @@ def test_tiled_ref_time_variant_bit_exact_vs_per_state_kernel():
     assert _bits_equal(a, b)
 
 
+def test_graph_replay_with_level_walk_n22():
+    """n=22 FP32 RK4 and the FP64 reference loop on tiles in the level walk:
+    the walk's table is built before the first captured chunk (16 steps at
+    n=22), and 40 steps = 2 replays + 8 direct steps equal direct launches
+    bit for bit."""
+    n = 22
+    r = np.random.default_rng(222).uniform(0.2, 1.5, n + 1)
+    x0 = _random_state(n, 223)
+    for prec, integ in (("f32", "rk4"), ("f64", "ref")):
+        out = []
+        for graph in (1, 0):
+            with T.Engine(n, prec, integ, "mf") as e:
+                e.set_option(T.OPT_GRAPH, graph)
+                e.set_rates(r)
+                e.set_x(x0)
+                info = e.run(0.0, 0.01, 40, patience=-1)
+                assert info.steps_done == 40
+                out.append(e.get_x())
+        assert _bits_equal(out[0], out[1]), (prec, integ)
+
+
 def test_graph_replay_does_not_change_results():
     """n=18: chunks of 256 steps are replayed from a captured graph; 600 steps
     = 2 replays + 88 direct steps, bitwise equal to direct launches."""
