@@ -150,7 +150,8 @@ int tasep_moments(tasep_ctx *ctx, double *y, double *cov, double *mul,
  * it is for callers that cannot allocate (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  x is a
  * caller-owned device vector of 2^n entries (double for TASEP_F64, float for
- * TASEP_F32), e.g. from tasep_get_x_device; results as tasep_moments (any
+ * TASEP_F32) on the current device, e.g. from tasep_get_x_device; results as
+ * tasep_moments (any
  * output pointer may be NULL), 2 <= n <= 34.  The block partials live in a
  * static device array of 1.17 MiB per GPU (256 blocks x 596 doubles plus
  * 596 sums) instead of allocated memory; every call on a GPU shares that

@@ -1289,6 +1289,9 @@ extern "C" int tasep_moments_noalloc(const void *x, int n, int precision, double
         (void)hipGetLastError();  // clear the lookup's error state
         return fail(TASEP_ERR_ARG, "the state vector is not device memory");
     }
+    int cur = 0;
+    HIP_OK(hipGetDevice(&cur));
+    if (attr.device != cur) return fail(TASEP_ERR_ARG, "the state vector is on another device than the current one");
     double *part = nullptr;
     int rc = scratch_on_device(&part);
     if (rc) return rc;
