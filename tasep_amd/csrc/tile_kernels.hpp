@@ -58,9 +58,11 @@ constexpr int kTileThreads = 512;
 constexpr int kTileWaves = kTileThreads / kWave;
 // Tile-bond streams issued per tile before its LDS phase (16 registers per
 // stream and lane); a tile with more active tile bonds reads the rest two at
-// a time afterwards.
+// a time afterwards.  Per step, one against two: RK4 FP32 n=28 2.825 /
+// 2.851 ms, n=29..31 0.8-1.0% faster, n=32, 33 and n=20..26 within +-1%;
+// RK4 FP64 n=28 5.893 / 5.947 ms, n=20 1.4% faster, n=22 3% slower.
 #ifndef TASEP_TILE_SLOTS
-#define TASEP_TILE_SLOTS 2
+#define TASEP_TILE_SLOTS 1
 #endif
 // Order within a tile: 1 = the prefetched own vector goes to LDS before the
 // tile's streams are issued, 0 = after, for FP32 and FP64 tiles.  Measured
