@@ -210,10 +210,12 @@ int tasep_step_profile(tasep_ctx *ctx, double t0, double h, int reps,
  *                              rates (default: the Horner form of the same
  *                              step, 3 work vectors instead of 5)
  *   TASEP_OPT_TILE_BLOCKS  [0] workgroups of the tiled passes (multiple of
- *                              8, at most 16384); 0 = 4 per workgroup slot
- *                              resident at once (1024 on 256 CUs), 16384 for
- *                              RK4 passes over at least 2^18 tiles, and at
- *                              most one per 8 tiles but at least one per CU
+ *                              8, at most 16384); 0 = one per workgroup slot
+ *                              resident at once (256 on 256 CUs) up to 2^15
+ *                              tiles (2^16 for the reference loop), 4 per
+ *                              slot above that, 16384 for RK4 passes over at
+ *                              least 2^18 tiles, and at most one per 8 tiles
+ *                              but at least one per CU
  *   TASEP_OPT_LDS          [1] matrix-free runs that keep two copies of the
  *                              state vector in one workgroup's LDS and run
  *                              all their steps there, in one launch per

@@ -534,8 +534,10 @@ static int device_cus() {
     return v > 0 ? v : 0;
 }
 
-// Grid and tile order of a tiled pass.  Grid: 4 workgroups per resident
-// slot (1024 on 256 CUs), at most kMaxBlocks partials for the end-of-step
+// Grid and tile order of a tiled pass.  Grid: one workgroup per resident
+// slot (256 on 256 CUs) up to 2^15 tiles (RK4 passes: FP32 n <= 28, FP64
+// n <= 27) or 2^16 tiles (reference loop: n <= 28), 4 per slot (1024) above
+// that, at most kMaxBlocks partials for the end-of-step
 // reduction, rounded down to a multiple of 8 XCDs; and at most ntiles/8
 // workgroups but no fewer than one per CU, so that where tiles are few
 // (mid n) each workgroup still walks several and the next tile's reads
@@ -548,8 +550,13 @@ static int device_cus() {
 // n=28 6.107 / 6.098 / 6.100 / 6.127 / 6.223 ms; n=30 within 1% for every
 // grid.  The RK4 passes (`wide`) over 2^18 tiles or more (FP32 n >= 31,
 // FP64 n >= 30) take kMaxBlocks workgroups (FP32 n=32: 60.2 ms per step
-// against 61.7 with 4096).  Order (tile_walk): the requested one where the
-// grid allows it, else the next simpler one.
+// against 61.7 with 4096).  One workgroup per slot against 1024, per step
+// (RK4 passes with one pre-issued tile bond): FP32 n=26 / 27 / 28 0.654 /
+// 1.345 / 2.820 ms against 0.674 / 1.364 / 2.830, n=29 6.08 against 6.03,
+// n=30 13.38 against 13.05; FP64 n=26 1.32 against 1.34, n=28 5.97 against
+// 5.92; reference loop n=26 / 27 / 28 0.319 / 0.670 / 1.391 ms against
+// 0.329 / 0.678 / 1.395, n=30 6.33 against 6.25.  Order (tile_walk): the
+// requested one where the grid allows it, else the next simpler one.
 // The level-walk table the context's passes will use, built ahead of a
 // graph capture: its host-to-device copy cannot be captured.  (Built for
 // the tile geometry of the context's precision, or the 64-row groups of the
@@ -625,6 +632,7 @@ static TileLaunch tile_launch(tasep_ctx *c, const void *kernel, int &slots_cache
     if (!slots_cache) slots_cache = resident_slots(kernel);
     const int slots = slots_cache;
     uint64_t cap = slots > 0 ? ((uint64_t)slots * 4) & ~(uint64_t)7 : 1024;
+    if (slots >= 8 && ntiles <= ((uint64_t)1 << (wide ? 15 : 16))) cap = (uint64_t)slots & ~(uint64_t)7;
     if (cap < 8) cap = 8;
     if (wide && ntiles >= ((uint64_t)1 << 18)) cap = kMaxBlocks;
     if (cap > (uint64_t)kMaxBlocks) cap = kMaxBlocks;
