@@ -851,11 +851,14 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
                 for (int j = 0; j < E; j++) Tv[c][j] += r * lv[j];
             }
         }
-        // ---- the reference update, checksum and mass (compensated over
-        // the per-column sums)
+        // ---- the reference update, checksum and mass (the lane's 8 states
+        // of the tile summed plain, then one compensated add per tile: per
+        // step at n=28 1.390 -> 1.379 ms, n=24 75 -> 71 us against one
+        // compensated add per column)
+        double cs = 0.0, ms = 0.0;
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            double o[E], cs = 0.0, ms = 0.0;
+            double o[E];
 #pragma unroll
             for (int j = 0; j < E; j++) {
                 const double d = ref_delta(h, Tv[c][j]);
@@ -863,10 +866,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_ref(const TileRefArgs a) 
                 cs += fabs(d);
                 ms += o[j];
             }
-            chk.add(cs);
-            mass.add(ms);
             st_out<NT, double, E>(chunk_rsrc(a.out + wb), lo + ((uint32_t)c << LB), o);
         }
+        chk.add(cs);
+        mass.add(ms);
         if (more) {
 #pragma unroll
             for (int c = 0; c < C; c++)
