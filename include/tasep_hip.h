@@ -213,9 +213,9 @@ int tasep_step_profile(tasep_ctx *ctx, double t0, double h, int reps,
  *                              8, at most 16384); 0 = one per workgroup slot
  *                              resident at once (256 on 256 CUs) up to 2^15
  *                              tiles (2^16 for the reference loop), 4 per
- *                              slot above that, 16384 for RK4 passes over at
- *                              least 2^18 tiles, and at most one per 8 tiles
- *                              but at least one per CU
+ *                              slot above that, 4096 / 16384 for RK4 passes
+ *                              over 2^17 / at least 2^18 tiles, and at most
+ *                              one per 8 tiles but at least one per CU
  *   TASEP_OPT_LDS          [1] matrix-free runs that keep two copies of the
  *                              state vector in one workgroup's LDS and run
  *                              all their steps there, in one launch per

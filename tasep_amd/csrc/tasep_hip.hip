@@ -550,7 +550,9 @@ static int device_cus() {
 // n=28 6.107 / 6.098 / 6.100 / 6.127 / 6.223 ms; n=30 within 1% for every
 // grid.  The RK4 passes (`wide`) over 2^18 tiles or more (FP32 n >= 31,
 // FP64 n >= 30) take kMaxBlocks workgroups (FP32 n=32: 60.2 ms per step
-// against 61.7 with 4096).  One workgroup per slot against 1024, per step
+// against 61.7 with 4096), over 2^17 tiles 4096 (per step with 1024 /
+// 2048 / 4096 / 16384: FP32 n=30 13.14 / 13.06 / 12.99 / 13.08 ms, FP64
+// n=29 13.07 / 12.98 / 12.89 / 12.94 ms; at 2^16 tiles 1024 and 2048 tie).  One workgroup per slot against 1024, per step
 // (RK4 passes with one pre-issued tile bond): FP32 n=26 / 27 / 28 0.654 /
 // 1.345 / 2.820 ms against 0.674 / 1.364 / 2.830, n=29 6.08 against 6.03,
 // n=30 13.38 against 13.05; FP64 n=26 1.32 against 1.34, n=28 5.97 against
@@ -634,7 +636,7 @@ static TileLaunch tile_launch(tasep_ctx *c, const void *kernel, int &slots_cache
     uint64_t cap = slots > 0 ? ((uint64_t)slots * 4) & ~(uint64_t)7 : 1024;
     if (slots >= 8 && ntiles <= ((uint64_t)1 << (wide ? 15 : 16))) cap = (uint64_t)slots & ~(uint64_t)7;
     if (cap < 8) cap = 8;
-    if (wide && ntiles >= ((uint64_t)1 << 18)) cap = kMaxBlocks;
+    if (wide && ntiles >= ((uint64_t)1 << 17)) cap = ntiles >= ((uint64_t)1 << 18) ? kMaxBlocks : 4096;
     if (cap > (uint64_t)kMaxBlocks) cap = kMaxBlocks;
     const int cus = device_cus();
     uint64_t few = ntiles / 8;
