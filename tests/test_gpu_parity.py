@@ -540,7 +540,7 @@ def test_config5_n32_rk4_f32_tiles_vs_f64_tiles_from_product_state():
 @pytest.mark.parametrize("n,steps", [(20, 30), (22, 12)])
 def test_tiled_f32_vs_oracle_from_product_state(n, steps):
     """FP32 tiled passes at n = 20, 22 (several tile bonds per tile, the
-    reads beyond the pre-issued two) against the FP64 oracle, from a product
+    reads beyond the pre-issued one) against the FP64 oracle, from a product
     state whose value depends on every bit: every element within 1e-4
     relative (a wrong partner moves an element by about h r, 1e-2
     relative)."""
@@ -602,6 +602,27 @@ def test_grid_16384_tile_orders_n27_rk4_f64():
     for o in outs[2:]:
         assert _bits_equal(o, outs[1])
     assert float(np.max(np.abs(outs[1] - outs[0]) / np.abs(outs[0]))) <= 1e-13
+
+
+@pytest.mark.parametrize("n,prec", [(28, "f32"), (30, "f32"), (29, "f64")])
+def test_default_grid_bitwise_vs_1024_workgroups(n, prec):
+    """The default grid of the RK4 tile passes -- one workgroup per CU up to
+    2^15 tiles (FP32 n=28: 128 tiles per workgroup, three walk-table
+    chunks), 4096 workgroups at 2^17 tiles (FP32 n=30, FP64 n=29) -- gives
+    the bits of 1024 workgroups: two steps from a product state whose value
+    depends on every bit; moments, window marginals and 12 state slices
+    spread over the vector compared bit for bit."""
+    r = np.random.default_rng(90 + n).uniform(0.2, 1.8, n + 1)
+    pb = _site_densities(n, 91 + n)
+    offs, width = _slices(1 << n, count=12)
+    a = _rk4_product_run(n, prec, r, pb, 2, offs, width)
+    b = _rk4_product_run(n, prec, r, pb, 2, offs, width, tile_blocks=1024)
+    for u, v in zip(a[:2], b[:2]):
+        assert _bits_equal(u, v)
+    assert a[2] == b[2]
+    assert _bits_equal(a[3], b[3])
+    for u, v in zip(a[4], b[4]):
+        assert _bits_equal(u, v)
 
 
 @pytest.mark.parametrize("prec,integ", [("f64", "ref"), ("f32", "rk4")])
