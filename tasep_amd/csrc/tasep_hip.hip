@@ -534,31 +534,30 @@ static int device_cus() {
     return v > 0 ? v : 0;
 }
 
-// Grid and tile order of a tiled pass.  Grid: one workgroup per resident
-// slot (256 on 256 CUs) up to 2^15 tiles (RK4 passes: FP32 n <= 28, FP64
-// n <= 27) or 2^16 tiles (reference loop: n <= 28), 4 per slot (1024) above
-// that, at most kMaxBlocks partials for the end-of-step
-// reduction, rounded down to a multiple of 8 XCDs; and at most ntiles/8
-// workgroups but no fewer than one per CU, so that where tiles are few
-// (mid n) each workgroup still walks several and the next tile's reads
-// overlap the current tile's LDS phase (per step, by grid: reference loop
-// FP64 n=22 / 23 / 24 24.3 / 44.9 / 89.3 us against 30.2 / 59.5 / 116.0
-// with one tile per workgroup; RK4 FP64 n=22..24 and FP32 n=23..25 12-16%
-// faster).  On the level walk, per step with 1024 / 2048 / 4096 / 8192 /
-// 16384 workgroups: RK4 FP32 n=28 3.056 / 3.070 / 3.105 / 3.162 / 3.252 ms,
-// reference loop n=28 1.513 / 1.521 / 1.538 / 1.571 / 1.638 ms, RK4 FP64
-// n=28 6.107 / 6.098 / 6.100 / 6.127 / 6.223 ms; n=30 within 1% for every
-// grid.  The RK4 passes (`wide`) over 2^18 tiles or more (FP32 n >= 31,
-// FP64 n >= 30) take kMaxBlocks workgroups (FP32 n=32: 60.2 ms per step
-// against 61.7 with 4096), over 2^17 tiles 4096 (per step with 1024 /
-// 2048 / 4096 / 16384: FP32 n=30 13.14 / 13.06 / 12.99 / 13.08 ms, FP64
-// n=29 13.07 / 12.98 / 12.89 / 12.94 ms; at 2^16 tiles 1024 and 2048 tie).  One workgroup per slot against 1024, per step
-// (RK4 passes with one pre-issued tile bond): FP32 n=26 / 27 / 28 0.654 /
-// 1.345 / 2.820 ms against 0.674 / 1.364 / 2.830, n=29 6.08 against 6.03,
-// n=30 13.38 against 13.05; FP64 n=26 1.32 against 1.34, n=28 5.97 against
-// 5.92; reference loop n=26 / 27 / 28 0.319 / 0.670 / 1.391 ms against
-// 0.329 / 0.678 / 1.395, n=30 6.33 against 6.25.  Order (tile_walk): the
-// requested one where the grid allows it, else the next simpler one.
+// Grid and tile order of a tiled pass.  Grid, by the number of tiles (the
+// kernels run one 512-lane workgroup per CU at a time, 256 resident slots):
+//   up to 2^15 tiles (2^16 for the reference loop)  one workgroup per slot
+//   above that                                      4 per slot (1024)
+//   RK4 passes (`wide`) over 2^17 tiles             4096
+//   RK4 passes over 2^18 tiles or more              kMaxBlocks (16384)
+// always at most ntiles/8 but at least one per CU, rounded down to a
+// multiple of the 8 XCDs, so that where tiles are few (mid n) each
+// workgroup still walks several and the next tile's reads overlap the
+// current tile's LDS phase (reference loop FP64 n=22 / 23 / 24 24.3 / 44.9 /
+// 89.3 us per step against 30.2 / 59.5 / 116.0 with one tile per
+// workgroup).  Measured per step:
+//   256 vs 1024      RK4 FP32 n=26 / 27 / 28 0.654 / 1.345 / 2.820 ms against
+//                    0.674 / 1.364 / 2.830, n=29 6.08 against 6.03, n=30
+//                    13.38 against 13.05; FP64 n=26 1.32 against 1.34, n=28
+//                    5.97 against 5.92; reference loop n=26 / 27 / 28 0.319 /
+//                    0.670 / 1.391 against 0.329 / 0.678 / 1.395, n=30 6.33
+//                    against 6.25
+//   1024 / 2048 /    RK4 FP32 n=30 13.14 / 13.06 / 12.99 / 13.08 ms, FP64 n=29
+//   4096 / 16384     13.07 / 12.98 / 12.89 / 12.94 ms; at 2^16 tiles 1024 and
+//                    2048 tie; FP32 n=32 60.2 ms with 16384 against 61.7 with
+//                    4096
+// Order (tile_walk): the requested one where the grid allows it, else the
+// next simpler one.
 // The level-walk table the context's passes will use, built ahead of a
 // graph capture: its host-to-device copy cannot be captured.  (Built for
 // the tile geometry of the context's precision, or the 64-row groups of the
