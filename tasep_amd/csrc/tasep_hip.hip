@@ -609,6 +609,15 @@ static void level_walk(int tb, std::vector<uint32_t> &walk) {
 
 static const uint32_t *ensure_walk(tasep_ctx *c, int tb) {
     if (c->walk_tb == tb) return c->walk;
+    // a table missing under a graph capture (its allocation failed before
+    // the capture) is not built there: the allocation and copy would end
+    // the capture; the passes take the order the table would permute
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (cs != hipStreamCaptureStatusNone) return nullptr;
     std::vector<uint32_t> w;
     level_walk(tb, w);
     // without the table the passes take the order it permutes; clear the
