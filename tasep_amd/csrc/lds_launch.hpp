@@ -3,6 +3,9 @@
 // integrator and rate kind.
 #pragma once
 
+#include <atomic>
+#include <mutex>
+
 #include "host_common.hpp"
 #include "lds_kernels.hpp"
 
@@ -13,7 +16,8 @@ using namespace tasep;
 // Largest dynamic LDS a workgroup may take on the current device (the
 // kernel's static LDS, about 4 KiB, is kept aside).
 inline size_t lds_limit() {
-    static size_t lim = 0;
+    static std::atomic<size_t> cache{0};
+    size_t lim = cache.load(std::memory_order_relaxed);
     if (!lim) {
         int dev = 0, v = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -21,6 +25,7 @@ inline size_t lds_limit() {
             lim = (size_t)v;
         else
             lim = 64 * 1024;
+        cache.store(lim, std::memory_order_relaxed);
     }
     return lim - 8192;
 }
@@ -40,12 +45,11 @@ inline bool lds_fits(int n, int esize, int integ) { return n <= kLdsMaxN && lds_
 
 template <class V, int INTEG, bool TV>
 inline void launch_lds_k(const LdsArgs &a, int blocks, int nt, size_t shmem, hipStream_t s) {
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr;
+    std::call_once(attr, [] {
         hipFuncSetAttribute((const void *)k_lds_run<V, INTEG, TV>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds_limit());
-        attr = true;
-    }
+    });
     hipLaunchKernelGGL((k_lds_run<V, INTEG, TV>), dim3(blocks), dim3(nt), shmem, s, a);
 }
 

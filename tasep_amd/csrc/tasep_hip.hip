@@ -637,10 +637,15 @@ static const uint32_t *ensure_walk(tasep_ctx *c, int tb) {
     c->walk_tb = tb;
     return d;
 }
-static TileLaunch tile_launch(tasep_ctx *c, const void *kernel, int &slots_cache, uint64_t ntiles,
+// slots_cache: the kernel's resident workgroups, looked up once (contexts on
+// other threads may launch the same kernel at the same time)
+static TileLaunch tile_launch(tasep_ctx *c, const void *kernel, std::atomic<int> &slots_cache, uint64_t ntiles,
                               bool wide = false) {
-    if (!slots_cache) slots_cache = resident_slots(kernel);
-    const int slots = slots_cache;
+    int slots = slots_cache.load(std::memory_order_relaxed);
+    if (!slots) {
+        slots = resident_slots(kernel);
+        slots_cache.store(slots, std::memory_order_relaxed);
+    }
     uint64_t cap = slots > 0 ? ((uint64_t)slots * 4) & ~(uint64_t)7 : 1024;
     if (slots >= 8 && ntiles <= ((uint64_t)1 << (wide ? 15 : 16))) cap = (uint64_t)slots & ~(uint64_t)7;
     if (cap < 8) cap = 8;
@@ -681,7 +686,7 @@ static TileLaunch tile_launch(tasep_ctx *c, const void *kernel, int &slots_cache
 
 template <class V, int EPI, bool SELF, bool NT>
 static uint32_t launch_tile_nt(tasep_ctx *c, TileArgs<V> a) {
-    static int slots = 0;
+    static std::atomic<int> slots{0};
     const TileLaunch L = tile_launch(c, (const void *)k_tile_pass<V, EPI, SELF, NT>, slots, a.ntiles, true);
     a.xcd_order = L.order;
     a.xcd_res = L.res;
@@ -783,7 +788,7 @@ static void ref_step(tasep_ctx *c, double h, uint64_t j) {
         t.h = h;
         t.n = c->n;
         t.ntiles = c->dim >> TileGeom<double>::b;
-        static int slots[2] = {0, 0};
+        static std::atomic<int> slots[2];
         const bool nt = nt_out(c);
         const void *kernel = nt ? (const void *)k_tile_ref<true> : (const void *)k_tile_ref<false>;
         const TileLaunch L = tile_launch(c, kernel, slots[nt], t.ntiles);
